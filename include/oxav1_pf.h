@@ -1,0 +1,395 @@
+/*
+ * oxav1_pf.h — C ABI of the MI355X (gfx950) AV1 post-reconstruction filter
+ * chain: §7.14 deblocking, §7.15 CDEF, §7.16 superres, §7.17 loop
+ * restoration and §7.18.3 film-grain synthesis.
+ *
+ * Two layers are exported:
+ *
+ *  1. Drop-in entry points (oxav1_pf_loop_filter_frame, _cdef_frame,
+ *     _upscale_frame, _loop_restoration_frame, _film_grain_synthesis).
+ *     Each replaces one public Rust function of oxideav-av1 with the same
+ *     argument order and meaning.  Planes are the reference's PlaneBuffer
+ *     (host int32 samples, stride = cols).  The per-block closures of the
+ *     reference contexts become dense grids sampled on the caller's thread.
+ *     The call stages the planes to HBM, runs the HIP kernels and returns
+ *     after the results are back in the caller's buffers.
+ *
+ *  2. Device-resident engine (oxav1_pf_engine_*): whole-chain jobs on
+ *     u8/u16 planes already in HBM, plain (one frame per launch) or batched
+ *     (one persistent launch per stage over many frames, hipGraph replay).
+ *
+ * Every function returns an int status (OXAV1_OK or an error code below)
+ * and never aborts.  No torch types appear in this interface.
+ */
+#ifndef OXAV1_PF_H
+#define OXAV1_PF_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes                                                            */
+/* ---------------------------------------------------------------------- */
+#define OXAV1_OK 0
+/* The three SuperresError variants (superres.rs:217-230). */
+#define OXAV1_ERR_UPSCALED_NOT_LARGER 1
+#define OXAV1_ERR_PLANE_SHAPE_MISMATCH 2
+#define OXAV1_ERR_PLANE_COUNT_MISMATCH 3
+/* Conditions the reference cannot express (it would panic or clamp). */
+#define OXAV1_ERR_INVALID_ARG 10
+#define OXAV1_ERR_ALIAS 11
+#define OXAV1_ERR_HIP 12
+#define OXAV1_ERR_OUT_OF_MEMORY 13
+#define OXAV1_ERR_ADMISSION_REFUSED 14
+#define OXAV1_ERR_UNSUPPORTED 15
+#define OXAV1_ERR_NO_DEVICE 16
+
+/* ---------------------------------------------------------------------- */
+/* Shared types                                                            */
+/* ---------------------------------------------------------------------- */
+
+/* PlaneBuffer (loop_filter.rs:119-126): row-major, stride = cols. */
+typedef struct oxav1_plane_i32 {
+    int32_t *samples;
+    uint32_t rows;
+    uint32_t cols;
+} oxav1_plane_i32;
+
+/* BLOCK_* ordinals (cdf.rs:14723-14770); 22 = BLOCK_INVALID. */
+#define OXAV1_BLOCK_INVALID 22
+/* FrameRestorationType (uncompressed_header_tail.rs:1019-1030). */
+#define OXAV1_RESTORE_NONE 0
+#define OXAV1_RESTORE_WIENER 1
+#define OXAV1_RESTORE_SGRPROJ 2
+#define OXAV1_RESTORE_SWITCHABLE 3
+
+/* ---------------------------------------------------------------------- */
+/* §7.14 deblocking                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Scalar fields of LoopFilterFrameContext (loop_filter.rs:167-201). */
+typedef struct oxav1_lf_ctx {
+    uint8_t loop_filter_level[4];
+    uint8_t loop_filter_sharpness;
+    uint8_t loop_filter_delta_enabled;
+    uint8_t delta_lf_multi;
+    uint8_t num_planes;
+    int8_t loop_filter_ref_deltas[8];
+    int8_t loop_filter_mode_deltas[2];
+    uint8_t bit_depth;
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    uint8_t reserved0;
+    uint32_t mi_rows;
+    uint32_t mi_cols;
+    uint32_t frame_width;  /* coded FrameWidth (frame_driver.rs:853) */
+    uint32_t frame_height;
+} oxav1_lf_ctx;
+
+/*
+ * The closures of LoopFilterFrameContext (loop_filter.rs:202-237) sampled
+ * into dense grids.  Per-mi grids are mi_rows*mi_cols, row-major.
+ *   is_intra(r,c), skip(r,c), ref_frame(r,c), mode(r,c), segment_id(r,c),
+ *   mi_size(r,c): one byte each.
+ *   delta_lf(r,c,idx): 4 signed bytes per mi cell; NULL means all zero.
+ *   segment_id: NULL means 255 (no segment) everywhere.
+ *   lf_tx_size(plane, r, c): plane p grid is ((mi_rows+sy)>>sy) rows by
+ *   ((mi_cols+sx)>>sx) cols, sx/sy = 0 for plane 0.
+ *   seg_feature_active/data(segment, feature): 8 segments x 8 features;
+ *   segment >= 8 reads inactive/0 (the bridge at cdf.rs:20514-20530).
+ */
+typedef struct oxav1_lf_grids {
+    const uint8_t *is_intra;
+    const uint8_t *skip;
+    const uint8_t *ref_frame;
+    const uint8_t *mode;
+    const uint8_t *segment_id;
+    const int8_t *delta_lf;
+    const uint8_t *mi_size;
+    const uint8_t *lf_tx_size[3];
+    uint8_t seg_feature_active[8][8];
+    int16_t seg_feature_data[8][8];
+} oxav1_lf_grids;
+
+/* Replaces loop_filter_frame (loop_filter.rs:271). In place on planes. */
+int oxav1_pf_loop_filter_frame(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *grids,
+                               oxav1_plane_i32 *planes, uint32_t num_plane_bufs);
+
+/* ---------------------------------------------------------------------- */
+/* §7.15 CDEF                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* CdefParams (uncompressed_header_tail.rs:876-900). */
+typedef struct oxav1_cdef_params {
+    uint8_t cdef_damping;
+    uint8_t cdef_bits;
+    uint8_t y_pri_strength[8];
+    uint8_t y_sec_strength[8];
+    uint8_t uv_pri_strength[8];
+    uint8_t uv_sec_strength[8];
+} oxav1_cdef_params;
+
+/* Scalar fields of CdefFrameContext (cdef.rs:114-135). */
+typedef struct oxav1_cdef_ctx {
+    uint32_t mi_rows;
+    uint32_t mi_cols;
+    uint8_t num_planes;
+    uint8_t bit_depth;
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    oxav1_cdef_params params;
+} oxav1_cdef_ctx;
+
+/*
+ * cdef_idx(r,c) is only queried at 64x64 anchors (cdef.rs:193-195), so it is
+ * one signed byte per 64x64 unit: ((mi_rows+15)>>4) x ((mi_cols+15)>>4).
+ * skip(r,c) is mi_rows*mi_cols bytes.
+ */
+typedef struct oxav1_cdef_grids {
+    const int8_t *cdef_idx;
+    const uint8_t *skip;
+} oxav1_cdef_grids;
+
+/* Replaces cdef_frame (cdef.rs:180). src and dst must not alias. */
+int oxav1_pf_cdef_frame(const oxav1_cdef_ctx *ctx, const oxav1_cdef_grids *grids,
+                        const oxav1_plane_i32 *src, uint32_t num_src,
+                        oxav1_plane_i32 *dst, uint32_t num_dst);
+
+/* ---------------------------------------------------------------------- */
+/* §7.16 superres                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* SuperresFrameContext (superres.rs:239-264). */
+typedef struct oxav1_sr_ctx {
+    uint8_t use_superres;
+    uint8_t num_planes;
+    uint8_t bit_depth;
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    uint8_t reserved[3];
+    uint32_t frame_width;
+    uint32_t upscaled_width;
+    uint32_t frame_height;
+    uint32_t mi_cols;
+} oxav1_sr_ctx;
+
+/* Replaces upscale_frame (superres.rs:282); returns the SuperresError codes.
+ * Like the reference, planes before a failing plane are already written. */
+int oxav1_pf_upscale_frame(const oxav1_sr_ctx *ctx, const oxav1_plane_i32 *in, uint32_t num_in,
+                           oxav1_plane_i32 *out, uint32_t num_out);
+
+/* ---------------------------------------------------------------------- */
+/* §7.17 loop restoration                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* One restoration unit: LrType / LrWiener / LrSgrSet / LrSgrXqd
+ * (loop_restoration.rs:217-233). wiener[pass][i], pass 0 = vertical. */
+typedef struct oxav1_lr_unit {
+    uint8_t type;    /* OXAV1_RESTORE_* */
+    uint8_t sgr_set; /* 0..15 */
+    int16_t wiener[2][3];
+    int16_t sgr_xqd[2];
+} oxav1_lr_unit;
+
+/* LoopRestorationFrameContext scalars + LrParams (loop_restoration.rs:194-216,
+ * uncompressed_header_tail.rs:1079-1109). */
+typedef struct oxav1_lr_ctx {
+    uint32_t mi_rows;
+    uint32_t mi_cols;
+    uint8_t num_planes;
+    uint8_t bit_depth;
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    uint32_t frame_height;
+    uint32_t upscaled_width;
+    uint8_t frame_restoration_type[3];
+    uint8_t uses_lr;
+    uint32_t loop_restoration_size[3];
+} oxav1_lr_ctx;
+
+/* Per-plane unit grids, unit_rows x unit_cols with
+ * unit_rows = count_units_in_frame(size, Round2(FrameHeight, sy)) and
+ * unit_cols = count_units_in_frame(size, Round2(UpscaledWidth, sx))
+ * (loop_restoration.rs:395-398). NULL = every unit RESTORE_NONE. */
+typedef struct oxav1_lr_grids {
+    const oxav1_lr_unit *units[3];
+} oxav1_lr_grids;
+
+/* Replaces loop_restoration_frame (loop_restoration.rs:268).
+ * curr = pre-CDEF (upscaled) planes, cdef = CDEF output, lr = output. */
+int oxav1_pf_loop_restoration_frame(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *grids,
+                                    const oxav1_plane_i32 *curr, uint32_t num_curr,
+                                    const oxav1_plane_i32 *cdef, uint32_t num_cdef,
+                                    oxav1_plane_i32 *lr, uint32_t num_lr);
+
+/* count_units_in_frame (loop_restoration.rs:910-915). */
+uint32_t oxav1_pf_count_units_in_frame(uint32_t unit_size, uint32_t frame_size);
+
+/* ---------------------------------------------------------------------- */
+/* §7.18.3 film grain                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* FilmGrainParams (uncompressed_header_tail.rs:1763-1834). grain_scaling is
+ * already +8 and ar_coeff_shift already +6, as in the reference. */
+typedef struct oxav1_film_grain_params {
+    uint8_t apply_grain;
+    uint8_t update_grain;
+    uint16_t grain_seed;
+    uint8_t num_y_points;
+    uint8_t point_y_value[14];
+    uint8_t point_y_scaling[14];
+    uint8_t chroma_scaling_from_luma;
+    uint8_t num_cb_points;
+    uint8_t point_cb_value[10];
+    uint8_t point_cb_scaling[10];
+    uint8_t num_cr_points;
+    uint8_t point_cr_value[10];
+    uint8_t point_cr_scaling[10];
+    uint8_t grain_scaling;
+    uint8_t ar_coeff_lag;
+    uint8_t ar_coeffs_y_plus_128[24];
+    uint8_t ar_coeffs_cb_plus_128[25];
+    uint8_t ar_coeffs_cr_plus_128[25];
+    uint8_t ar_coeff_shift;
+    uint8_t grain_scale_shift;
+    uint8_t cb_mult;
+    uint8_t cb_luma_mult;
+    uint16_t cb_offset;
+    uint8_t cr_mult;
+    uint8_t cr_luma_mult;
+    uint16_t cr_offset;
+    uint8_t overlap_flag;
+    uint8_t clip_to_restricted_range;
+} oxav1_film_grain_params;
+
+/* Replaces film_grain_synthesis (film_grain.rs:848). In place on planes
+ * (the cropped output planes; planes[0] gives w/h as in film_grain.rs:585). */
+int oxav1_pf_film_grain_synthesis(const oxav1_film_grain_params *fg, uint8_t bit_depth,
+                                  uint8_t num_planes, uint8_t subsampling_x,
+                                  uint8_t subsampling_y, uint8_t matrix_coefficients,
+                                  oxav1_plane_i32 *planes, uint32_t num_plane_bufs);
+
+/* ---------------------------------------------------------------------- */
+/* Library / device                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Initialise the HIP runtime on the current device. Optional: every entry
+ * point calls it lazily. Returns OXAV1_ERR_NO_DEVICE without a GPU. */
+int oxav1_pf_init(void);
+/* Human-readable text for a status code (static storage). */
+const char *oxav1_pf_strerror(int status);
+/* Text of the last HIP error seen by this thread (static per-thread). */
+const char *oxav1_pf_last_hip_error(void);
+/* Sum of (i+1)*value over one of the library's constant tables (names as in
+ * the reference: "GAUSSIAN_SEQUENCE", "UPSCALE_FILTER", "SGR_PARAMS", ...);
+ * INT64_MIN for an unknown name. */
+int64_t oxav1_pf_table_checksum(const char *name);
+/* sizeof of an ABI struct by its type name ("oxav1_lf_ctx", ...), 0 if
+ * unknown; lets foreign bindings verify their layouts. */
+size_t oxav1_pf_struct_size(const char *name);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident chain engine                                            */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Packed per-mi side information for the deblocking kernels, one uint32 per
+ * mi cell (the LoopFilterFrameContext closures in a single word):
+ *   bits 0-4 mi_size, bit 5 skip, bit 6 is_intra, bits 8-11 ref_frame,
+ *   bits 12-16 mode, bits 24-31 segment_id.
+ */
+#define OXAV1_MI_SIZE_SHIFT 0
+#define OXAV1_MI_SKIP_BIT 5
+#define OXAV1_MI_INTRA_BIT 6
+#define OXAV1_MI_REF_SHIFT 8
+#define OXAV1_MI_MODE_SHIFT 12
+#define OXAV1_MI_SEG_SHIFT 24
+
+/* Frame-level parameters of one chain job (the gates of
+ * frame_driver.rs:832-1096 plus the per-stage parameter structs). */
+typedef struct oxav1_frame_desc {
+    uint32_t frame_width;    /* coded (post-superres-downscale) width */
+    uint32_t frame_height;
+    uint32_t upscaled_width; /* == frame_width without superres */
+    uint32_t mi_rows;
+    uint32_t mi_cols;
+    uint8_t bit_depth;
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    uint8_t num_planes;
+    uint8_t matrix_coefficients;
+    uint8_t use_superres;
+    uint8_t apply_lf;   /* (lvl[0]||lvl[1]) && !coded_lossless && !allow_intrabc */
+    uint8_t apply_cdef; /* !coded_lossless && !allow_intrabc && enable_cdef */
+    oxav1_lf_ctx lf;
+    oxav1_cdef_params cdef;
+    oxav1_lr_ctx lr;
+    oxav1_film_grain_params fg;
+    uint8_t seg_feature_active[8][8];
+    int16_t seg_feature_data[8][8];
+} oxav1_frame_desc;
+
+/* Device pointers of one job. Planes are u8 (8-bit) or u16 (10/12-bit);
+ * pitches are in samples.  src is the padded pre-filter CurrFrame
+ * ((mi_rows*4)>>sy x (mi_cols*4)>>sx), ref_out the cropped pre-grain frame
+ * (UpscaledWidth x FrameHeight, subsampled), out the film-grain output
+ * (may equal ref_out's buffers only when film grain is off). */
+typedef struct oxav1_frame_dev {
+    const void *src[3];
+    uint32_t src_pitch[3];
+    void *ref_out[3];
+    uint32_t ref_pitch[3];
+    void *out[3];
+    uint32_t out_pitch[3];
+    const uint32_t *lf_mi;       /* mi_rows*mi_cols packed words */
+    const int8_t *delta_lf;      /* mi_rows*mi_cols*4 or NULL */
+    const uint8_t *lf_tx_size[3];
+    const int8_t *cdef_idx;      /* per 64x64 unit */
+    const oxav1_lr_unit *lr_units[3];
+} oxav1_frame_dev;
+
+typedef struct oxav1_pf_engine oxav1_pf_engine;
+
+/* Create an engine bound to the current device and a HIP stream (NULL =
+ * the engine creates its own default-priority stream). max_frames bounds
+ * a batch; intermediate buffers are allocated per frame slot by
+ * oxav1_pf_engine_prepare and grow on demand. */
+int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, uint32_t max_frames);
+int oxav1_pf_engine_destroy(oxav1_pf_engine *e);
+/* Admit a batch of n frames: validate, size the per-frame intermediates
+ * (deblocked, CDEF, upscaled planes, film-grain scratch) and upload the
+ * device job table.  Synchronous.  Returns OXAV1_ERR_ADMISSION_REFUSED when
+ * n exceeds max_frames and OXAV1_ERR_OUT_OF_MEMORY when HBM is exhausted. */
+int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
+                            const oxav1_frame_dev *dev, uint32_t n);
+/* Run the prepared batch, asynchronously on the engine's stream.
+ * mode 0: plain, one launch per stage per frame;
+ * mode 1: batched, one persistent launch per stage pulling
+ *         (frame, plane, tile) items from a device counter;
+ * mode 2: mode 1 captured once in a hipGraph and replayed. */
+int oxav1_pf_engine_run(oxav1_pf_engine *e, int mode);
+/* prepare(n = 1) + run(mode 0). */
+int oxav1_pf_engine_run_frame(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
+                              const oxav1_frame_dev *dev);
+/* prepare(n) + run(use_graph ? 2 : 1). */
+int oxav1_pf_engine_run_batch(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
+                              const oxav1_frame_dev *dev, uint32_t n, int use_graph);
+/* The engine's HIP stream (hipStream_t). */
+void *oxav1_pf_engine_stream(oxav1_pf_engine *e);
+/* Wait for all work on the engine's stream. */
+int oxav1_pf_engine_sync(oxav1_pf_engine *e);
+/* Time the dominant kernel of the last runs: average duration in ms of the
+ * named stage ("lf","cdef","sr","lr","fg") measured with HIP events on the
+ * engine's stream when timing is enabled. */
+int oxav1_pf_engine_set_timing(oxav1_pf_engine *e, int enable);
+int oxav1_pf_engine_stage_ms(oxav1_pf_engine *e, const char *stage, double *avg_ms,
+                             uint32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OXAV1_PF_H */

@@ -1,0 +1,129 @@
+/*
+ * oxav1_oracle.h — CPU oracle for the AV1 post-reconstruction filter chain.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's CPU filter path (oxideav-av1 v0.1.16: src/loop_filter.rs,
+ * src/cdef.rs, src/superres.rs, src/loop_restoration.rs, src/film_grain.rs
+ * and the chain in src/decoder/frame_driver.rs:790-1116).  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker or the timed CPU baseline.  The product library never
+ * links or calls it.
+ *
+ * The closure-based contexts of the reference are taken as the same dense
+ * grids the product's C ABI uses (include/oxav1_pf.h); each grid accessor
+ * below mirrors the closure the reference's walker bridges build
+ * (cdf.rs:20320-20678).
+ *
+ * Parity status of each function is recorded in oxav1_oracle.c's header.
+ */
+#ifndef OXAV1_ORACLE_H
+#define OXAV1_ORACLE_H
+
+#include "../include/oxav1_pf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- §7.14 deblocking (loop_filter.rs) ---- */
+typedef struct oo_lf_strength {
+    uint8_t lvl, limit, blimit, thresh;
+} oo_lf_strength;
+typedef struct oo_lf_masks {
+    uint8_t hev_mask, filter_mask, flat_mask, flat_mask2;
+} oo_lf_masks;
+
+void oo_loop_filter_frame(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                          oxav1_plane_i32 *planes, uint32_t n);
+void oo_loop_filter_edge(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                         oxav1_plane_i32 *planes, uint32_t n, uint8_t plane, uint8_t pass,
+                         uint32_t row, uint32_t col);
+uint32_t oo_lf_filter_size(uint32_t tx_sz, uint32_t prev_tx_sz, uint8_t pass, uint8_t plane);
+oo_lf_strength oo_lf_adaptive_filter_strength(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                                              uint32_t row, uint32_t col, uint8_t plane,
+                                              uint8_t pass);
+oo_lf_masks oo_lf_filter_mask(const oxav1_plane_i32 *buf, int32_t x, int32_t y, uint8_t plane,
+                              uint8_t limit, uint8_t blimit, uint8_t thresh, int32_t dx,
+                              int32_t dy, uint32_t filter_size, uint8_t bit_depth);
+void oo_lf_narrow_filter(oxav1_plane_i32 *buf, int32_t x, int32_t y, int32_t dx, int32_t dy,
+                         int hev_mask, uint8_t bit_depth);
+void oo_lf_wide_filter(oxav1_plane_i32 *buf, int32_t x, int32_t y, uint8_t plane, int32_t dx,
+                       int32_t dy, uint32_t log2_size);
+int32_t oo_get_plane_residual_size(uint32_t subsize, uint8_t plane, uint8_t sx, uint8_t sy);
+int32_t oo_get_tx_size(uint8_t plane, uint32_t tx_size, uint32_t mi_size, uint8_t sx, uint8_t sy);
+
+/* ---- §7.15 CDEF (cdef.rs) ---- */
+void oo_cdef_frame(const oxav1_cdef_ctx *ctx, const oxav1_cdef_grids *g,
+                   const oxav1_plane_i32 *src, uint32_t nsrc, oxav1_plane_i32 *dst,
+                   uint32_t ndst);
+void oo_cdef_direction(const oxav1_cdef_ctx *ctx, const oxav1_plane_i32 *src, uint32_t nsrc,
+                       uint32_t r, uint32_t c, int32_t *y_dir, int32_t *var);
+void oo_cdef_filter_block(const oxav1_cdef_ctx *ctx, const oxav1_plane_i32 *src, uint32_t nsrc,
+                          oxav1_plane_i32 *dst, uint32_t ndst, uint8_t plane, uint32_t r,
+                          uint32_t c, int32_t pri_str, int32_t sec_str, int32_t damping,
+                          int32_t dir);
+int32_t oo_cdef_constrain(int32_t diff, int32_t threshold, int32_t damping);
+
+/* ---- §7.16 superres (superres.rs) ---- */
+int oo_upscale_frame(const oxav1_sr_ctx *ctx, const oxav1_plane_i32 *in, uint32_t nin,
+                     oxav1_plane_i32 *out, uint32_t nout);
+int32_t oo_upscale_sample(const int32_t *row, uint32_t row_len, int32_t src_x_px,
+                          uint32_t subpel, int32_t min_x, int32_t max_x, uint8_t bit_depth);
+
+/* ---- §7.17 loop restoration (loop_restoration.rs) ---- */
+typedef struct oo_lr_geom {
+    uint32_t unit_row, unit_col, x, y, w, h;
+    int32_t stripe_start_y, stripe_end_y, plane_end_x, plane_end_y;
+} oo_lr_geom;
+void oo_loop_restoration_frame(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *g,
+                               const oxav1_plane_i32 *curr, uint32_t ncurr,
+                               const oxav1_plane_i32 *cdef, uint32_t ncdef,
+                               oxav1_plane_i32 *lr, uint32_t nlr);
+uint32_t oo_count_units_in_frame(uint32_t unit_size, uint32_t frame_size);
+oo_lr_geom oo_lr_derive_block_geometry(const oxav1_lr_ctx *ctx, uint8_t plane, uint32_t row,
+                                       uint32_t col);
+int32_t oo_lr_get_source_sample(const oxav1_plane_i32 *curr, uint32_t ncurr,
+                                const oxav1_plane_i32 *cdef, uint32_t ncdef, uint8_t plane,
+                                int32_t x, int32_t y, const oo_lr_geom *geom);
+void oo_wiener_coefficients(const int32_t coeff[3], int32_t filter[7]);
+/* box_filter (loop_restoration.rs:594-729); out has geom->h*geom->w entries. */
+void oo_lr_box_filter(const oxav1_plane_i32 *curr, uint32_t ncurr, const oxav1_plane_i32 *cdef,
+                      uint32_t ncdef, uint8_t plane, const oo_lr_geom *geom, uint8_t bit_depth,
+                      int32_t r, int32_t eps, uint8_t pass, int32_t *out);
+
+/* ---- §7.18.3 film grain (film_grain.rs) ---- */
+uint32_t oo_fg_random(uint16_t *state, uint32_t bits);
+/* luma: 73*82; cb/cr: chroma_h*chroma_w (<= 73*82). Returns chroma_w|chroma_h<<16. */
+uint32_t oo_fg_generate_grain(const oxav1_film_grain_params *fg, uint8_t bit_depth, uint8_t sx,
+                              uint8_t sy, int32_t *luma, int32_t *cb, int32_t *cr);
+void oo_fg_scaling_lookup_init(const oxav1_film_grain_params *fg, uint8_t num_planes,
+                               uint8_t lut[3][256]);
+int32_t oo_fg_scale_lut(const uint8_t lut[3][256], uint8_t plane, int32_t index,
+                        uint8_t bit_depth);
+void oo_film_grain_synthesis(const oxav1_film_grain_params *fg, uint8_t bit_depth,
+                             uint8_t num_planes, uint8_t sx, uint8_t sy,
+                             uint8_t matrix_coefficients, oxav1_plane_i32 *planes, uint32_t n);
+
+/* ---- Chain (frame_driver.rs:790-1116) ---- */
+/*
+ * Runs deblock -> CDEF -> superres -> LR -> crop -> (store) -> film grain on
+ * the padded pre-filter planes `rec` (modified: they end as the deblocked
+ * frame).  ref_out receives the cropped pre-grain planes, out the planes
+ * after film grain (both UpscaledWidth x FrameHeight, subsampled).
+ * stage_ns (optional, 5 entries) receives per-stage wall time.
+ */
+typedef struct oo_chain_grids {
+    oxav1_lf_grids lf;
+    oxav1_cdef_grids cdef;
+    oxav1_lr_grids lr;
+} oo_chain_grids;
+int oo_chain(const oxav1_frame_desc *desc, const oo_chain_grids *g, oxav1_plane_i32 *rec,
+             oxav1_plane_i32 *ref_out, oxav1_plane_i32 *out, double *stage_ns);
+
+/* Table checksum for the table tests: sum of (i+1)*value over the table. */
+int64_t oo_table_checksum(const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,314 @@
+// §7.15 CDEF on gfx950: one workgroup per 64x64 filter unit (all planes).
+//
+// Reference: cdef.rs:180-606.  CDEF reads only the deblocked frame and
+// writes a separate frame (cdef.rs:231-251), so 8x8 blocks are independent.
+// The workgroup stages the unit's luma and chroma source tiles with their
+// 2-sample borders in LDS, runs the §7.15.2 direction search with one lane
+// per 8x8 block, then filters every sample with the block's strengths.
+// Tap availability is "inside the plane buffer" (cdef.rs:501-511, 576-583);
+// units whose bordered tile lies inside the plane skip the per-tap checks.
+#pragma once
+
+#include "common.hpp"
+
+namespace oxav1 {
+
+constexpr int kCdefUnit = 64;           // luma samples per unit side
+constexpr int kCdefB = 2;               // tap reach
+constexpr int kCdefLw = kCdefUnit + 2 * kCdefB;  // 68
+
+__constant__ int8_t c_cdef_dir[8][2][2] = {
+    {{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}}, {{0, 1}, {1, 2}},
+    {{1, 1}, {2, 2}},   {{1, 0}, {2, 1}},  {{1, 0}, {2, 0}}, {{1, 0}, {2, -1}}};
+__constant__ uint8_t c_cdef_uv_dir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
+                                               {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
+__constant__ int32_t c_div_table[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+
+// Per 8x8 block decisions shared by the unit's threads.
+struct CdefBlk {
+    int8_t dir_y, dir_uv;
+    uint8_t active;  // 0: copy only
+    uint8_t pad;
+    int16_t pri_y, sec_y, pri_uv, sec_uv;
+};
+
+// cdef_direction (cdef.rs:366-449) on an 8x8 block held in LDS.
+template <typename L>
+__device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *dir_out, int32_t *var_out) {
+    int32_t x[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[i][j] = ((int32_t)t[i * stride + j] >> shift) - 128;
+    int32_t cost[8];
+    // directions 2 and 6
+    {
+        int32_t c2 = 0, c6 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            int32_t s = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) s += x[i][j];
+            c2 += s * s;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            int32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) s += x[i][j];
+            c6 += s * s;
+        }
+        cost[2] = c2 * 105;
+        cost[6] = c6 * 105;
+    }
+    // directions 0 and 4
+    {
+        int32_t p0[15], p4[15];
+#pragma unroll
+        for (int k = 0; k < 15; k++) p0[k] = p4[k] = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                p0[i + j] += x[i][j];
+                p4[7 + i - j] += x[i][j];
+            }
+        int32_t c0 = 0, c4 = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+            c0 += (p0[i] * p0[i] + p0[14 - i] * p0[14 - i]) * c_div_table[i + 1];
+            c4 += (p4[i] * p4[i] + p4[14 - i] * p4[14 - i]) * c_div_table[i + 1];
+        }
+        c0 += p0[7] * p0[7] * 105;
+        c4 += p4[7] * p4[7] * 105;
+        cost[0] = c0;
+        cost[4] = c4;
+    }
+    // directions 1, 3, 5, 7
+    {
+        int32_t p1[11], p3[11], p5[11], p7[11];
+#pragma unroll
+        for (int k = 0; k < 11; k++) p1[k] = p3[k] = p5[k] = p7[k] = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                p1[i + j / 2] += x[i][j];
+                p3[3 + i - j / 2] += x[i][j];
+                p5[3 - i / 2 + j] += x[i][j];
+                p7[i / 2 + j] += x[i][j];
+            }
+        int32_t *ps[4] = {p1, p3, p5, p7};
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int32_t *p = ps[d];
+            int32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < 5; j++) c += p[3 + j] * p[3 + j];
+            c *= 105;
+#pragma unroll
+            for (int j = 0; j < 3; j++) c += (p[j] * p[j] + p[10 - j] * p[10 - j]) * c_div_table[2 * j + 2];
+            cost[2 * d + 1] = c;
+        }
+    }
+    int32_t best = 0, dir = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (cost[k] > best) {
+            best = cost[k];
+            dir = k;
+        }
+    int32_t opp = cost[0];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (k == ((dir + 4) & 7)) opp = cost[k];
+    *dir_out = dir;
+    *var_out = (best - opp) >> 10;
+}
+
+// constrain (cdef.rs:592-606) with the per-block damping adjustment
+// precomputed (adj capped at 31, which shifts any |diff| < 2^31 to 0).
+__device__ __forceinline__ int32_t cdef_constrain(int32_t diff, int32_t thr, int32_t adj) {
+    const int32_t ad = abs(diff);
+    const int32_t v = min(ad, max(thr - (ad >> adj), 0));
+    return diff < 0 ? -v : v;
+}
+__device__ __forceinline__ int32_t cdef_adj(int32_t damping, int32_t thr) {
+    if (thr == 0) return 0;
+    return min(max(0, damping - (int32_t)floor_log2((uint32_t)thr)), 31);
+}
+
+// Filter one plane of a unit from an LDS tile with a kCdefB border.
+// CHECK: per-tap "inside the plane" test (units touching the frame edge).
+template <typename L, typename T, bool CHECK>
+__device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
+                                  int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefBlk *blk, int blk_stride,
+                                  bool chroma, int32_t damping, int32_t coeff_shift, T *__restrict__ dst,
+                                  uint32_t dpitch) {
+    for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
+        const int yy = i / tw, xx = i - yy * tw;
+        const int32_t py = oy + yy, px = ox + xx;
+        if (py >= ph || px >= pw) continue;
+        const CdefBlk &b = blk[(yy >> bh_log2) * blk_stride + (xx >> bw_log2)];
+        const L *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
+        const int32_t x = (int32_t)c[0];
+        if (!b.active) {
+            dst[(size_t)py * dpitch + px] = (T)x;
+            continue;
+        }
+        const int32_t pri = chroma ? b.pri_uv : b.pri_y;
+        const int32_t sec = chroma ? b.sec_uv : b.sec_y;
+        const int32_t dir = chroma ? b.dir_uv : b.dir_y;
+        const int32_t pri_adj = cdef_adj(damping, pri), sec_adj = cdef_adj(damping, sec);
+        const int pri_row = (pri >> coeff_shift) & 1;
+        const int32_t pt0 = pri_row ? 3 : 4, pt1 = pri_row ? 3 : 2;
+        int32_t sum = 0, mx = x, mn = x;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+#pragma unroll
+            for (int si = 0; si < 2; si++) {
+                const int32_t sgn = si ? 1 : -1;
+                {
+                    const int32_t dy = sgn * c_cdef_dir[dir][k][0], dx = sgn * c_cdef_dir[dir][k][1];
+                    if (!CHECK || (py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) {
+                        const int32_t v = (int32_t)c[dy * ls + dx];
+                        sum += (k ? pt1 : pt0) * cdef_constrain(v - x, pri, pri_adj);
+                        mx = max(mx, v);
+                        mn = min(mn, v);
+                    }
+                }
+#pragma unroll
+                for (int oi = 0; oi < 2; oi++) {
+                    const int32_t d2 = (dir + (oi ? 2 : -2)) & 7;
+                    const int32_t dy = sgn * c_cdef_dir[d2][k][0], dx = sgn * c_cdef_dir[d2][k][1];
+                    if (!CHECK || (py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) {
+                        const int32_t v = (int32_t)c[dy * ls + dx];
+                        sum += (k ? 1 : 2) * cdef_constrain(v - x, sec, sec_adj);
+                        mx = max(mx, v);
+                        mn = min(mn, v);
+                    }
+                }
+            }
+        }
+        const int32_t delta = (8 + sum - (sum < 0 ? 1 : 0)) >> 4;
+        dst[(size_t)py * dpitch + px] = (T)clip3(mn, mx, x + delta);
+    }
+}
+
+template <typename L, typename T>
+__device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
+                          const T *__restrict__ src, uint32_t spitch) {
+    const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
+    for (int i = threadIdx.x; i < ls * lh; i += blockDim.x) {
+        const int yy = i / ls, xx = i - yy * ls;
+        const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
+        // Out-of-plane entries are never used as taps; clamp keeps the load in bounds.
+        const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
+        t[i] = (L)src[(size_t)yc * spitch + xc];
+    }
+}
+
+// One 64x64 unit (ur, uc in 64-luma units): deb -> cdf for all planes.
+template <typename T>
+__device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
+                          CdefBlk *blk) {
+    using L = typename LdsOf<T>::type;
+    const int tid = threadIdx.x;
+    const uint32_t r0 = ur * 16, c0 = uc * 16;  // mi coordinates
+    const int32_t coeff_shift = (int32_t)J.bd - 8;
+    const int8_t idx = J.cdef_idx[(size_t)ur * J.cdef_ucols + uc];
+    const bool do_cdef = J.apply_cdef != 0;
+    // luma tile
+    // The 8x8 walk writes min(buffer, walked extent) (cdef.rs:185-200, 320-352);
+    // taps and the direction search see the whole buffer.
+    const int32_t ph0 = (int32_t)J.pad_rows[0], pw0 = (int32_t)J.pad_cols[0];
+    const int32_t eh0 = (int32_t)J.cdef_ext_rows[0], ew0 = (int32_t)J.cdef_ext_cols[0];
+    const int32_t oy0 = (int32_t)r0 * 4, ox0 = (int32_t)c0 * 4;
+    const int tw0 = min(kCdefUnit, ew0 - ox0), th0 = min(kCdefUnit, eh0 - oy0);
+    if (tw0 <= 0 || th0 <= 0) return;
+    // staged extent: the unit inside the buffer (>= the walked extent)
+    const int lw0 = min(kCdefUnit, pw0 - ox0), lh0 = min(kCdefUnit, ph0 - oy0);
+    const int ls0 = lw0 + 2 * kCdefB;
+    L *t0 = lds;
+    cdef_load<L, T>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
+    __syncthreads();
+    // per 8x8 block: skip test + direction (one lane per block)
+    const int nbx = (tw0 + 7) >> 3, nby = (th0 + 7) >> 3;
+    if (tid < 64) {
+        const int by = tid >> 3, bx = tid & 7;
+        CdefBlk b{};
+        if (by < nby && bx < nbx && do_cdef && idx >= 0) {
+            const uint32_t r = r0 + 2 * by, c = c0 + 2 * bx;
+            auto skip_at = [&](uint32_t rr, uint32_t cc) -> bool {
+                return (J.lf_mi[(size_t)rr * J.mi_cols + cc] >> OXAV1_MI_SKIP_BIT) & 1u;
+            };
+            // cdef.rs:266-272: out-of-grid cells count as skip
+            const bool skip = skip_at(r, c) && (r + 1 >= J.mi_rows || skip_at(r + 1, c)) &&
+                              (c + 1 >= J.mi_cols || skip_at(r, c + 1)) &&
+                              (r + 1 >= J.mi_rows || c + 1 >= J.mi_cols || skip_at(r + 1, c + 1));
+            if (r < J.mi_rows && c < J.mi_cols && !skip) {
+                // direction over the 8x8 luma block, reads clamped to the buffer (cdef.rs:386-391)
+                L blkv[64];
+                for (int i = 0; i < 8; i++)
+                    for (int j = 0; j < 8; j++) {
+                        const int32_t yy = min(8 * by + i, lh0 - 1);
+                        const int32_t xx = min(8 * bx + j, lw0 - 1);
+                        blkv[i * 8 + j] = t0[(yy + kCdefB) * ls0 + xx + kCdefB];
+                    }
+                int32_t ydir, var;
+                cdef_find_dir<L>(blkv, 8, coeff_shift, &ydir, &var);
+                const int32_t pri_y = (int32_t)J.cdef.y_pri_strength[idx] << coeff_shift;
+                const int32_t sec_y = (int32_t)J.cdef.y_sec_strength[idx] << coeff_shift;
+                const int32_t var_str = (var >> 6) > 0 ? min((int32_t)floor_log2((uint32_t)(var >> 6)), 12) : 0;
+                b.active = 1;
+                b.dir_y = (int8_t)(pri_y == 0 ? 0 : ydir);
+                b.pri_y = (int16_t)(var != 0 ? (pri_y * (4 + var_str) + 8) >> 4 : 0);
+                b.sec_y = (int16_t)sec_y;
+                const int32_t pri_uv = (int32_t)J.cdef.uv_pri_strength[idx] << coeff_shift;
+                b.pri_uv = (int16_t)pri_uv;
+                b.sec_uv = (int16_t)((int32_t)J.cdef.uv_sec_strength[idx] << coeff_shift);
+                b.dir_uv = (int8_t)(pri_uv == 0 ? 0 : c_cdef_uv_dir[J.sx ? 1 : 0][J.sy ? 1 : 0][ydir]);
+            }
+        }
+        blk[tid] = b;
+    }
+    __syncthreads();
+    // luma filter
+    {
+        const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
+        const int32_t damping = (int32_t)J.cdef.cdef_damping + coeff_shift;
+        T *dst = plane_ptr<T>(J.cdf[0]);
+        if (inside)
+            cdef_filter_plane<L, T, false>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, blk, 8, false, damping,
+                                           coeff_shift, dst, J.cdf_pitch[0]);
+        else
+            cdef_filter_plane<L, T, true>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, blk, 8, false, damping,
+                                          coeff_shift, dst, J.cdf_pitch[0]);
+    }
+    if (J.np < 2) return;
+    // chroma planes (cdef.rs:296-313), each staged in turn
+    const int32_t sx = J.sx, sy = J.sy;
+    const int32_t damping_uv = (int32_t)J.cdef.cdef_damping + coeff_shift - 1;
+    for (int p = 1; p < 3; p++) {
+        const int32_t ph = (int32_t)J.pad_rows[p], pw = (int32_t)J.pad_cols[p];
+        const int32_t eh = (int32_t)J.cdef_ext_rows[p], ew = (int32_t)J.cdef_ext_cols[p];
+        const int32_t oy = oy0 >> sy, ox = ox0 >> sx;
+        const int tw = min(kCdefUnit >> sx, ew - ox), th = min(kCdefUnit >> sy, eh - oy);
+        const int lw = min(kCdefUnit >> sx, pw - ox), lh = min(kCdefUnit >> sy, ph - oy);
+        __syncthreads();
+        if (tw <= 0 || th <= 0) continue;
+        cdef_load<L, T>(t0, lw, lh, ox, oy, pw, ph, plane_ptr<T>(J.deb[p]), J.deb_pitch[p]);
+        __syncthreads();
+        const int ls = lw + 2 * kCdefB;
+        const bool inside = ox >= kCdefB && oy >= kCdefB && ox + tw + kCdefB <= pw && oy + th + kCdefB <= ph;
+        T *dst = plane_ptr<T>(J.cdf[p]);
+        if (inside)
+            cdef_filter_plane<L, T, false>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, blk, 8, true, damping_uv,
+                                           coeff_shift, dst, J.cdf_pitch[p]);
+        else
+            cdef_filter_plane<L, T, true>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, blk, 8, true, damping_uv,
+                                          coeff_shift, dst, J.cdf_pitch[p]);
+    }
+}
+
+}  // namespace oxav1

@@ -1,0 +1,131 @@
+// Shared device-side definitions for the gfx950 post-filter kernels.
+//
+// FrameJob is the device-resident description of one frame moving through
+// the chain: geometry, per-stage parameters, side-information pointers and
+// the HBM buffers of every stage.  The host builds it (engine.hip) and the
+// stage kernels read it from a device table, so one kernel launch can cover
+// many frames of different size, bit depth and subsampling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "../../include/oxav1_pf.h"
+
+namespace oxav1 {
+
+// Sample storage types: u8 (8-bit), u16 (10/12-bit) for the engine, i32 for
+// the drop-in entry points (the reference's PlaneBuffer samples).
+enum SampleKind : uint8_t { kU8 = 0, kU16 = 1, kI32 = 2 };
+
+template <typename T>
+struct LdsOf {
+    using type = uint16_t;
+};
+template <>
+struct LdsOf<int32_t> {
+    using type = int32_t;
+};
+
+struct FgScratch;  // film-grain templates / LUT / offsets (fg.hpp)
+
+struct FrameJob {
+    // ---- geometry ----
+    uint32_t fw, fh, uw, mi_rows, mi_cols;
+    uint8_t bd, sx, sy, np;
+    uint8_t mc, use_sr, apply_lf, apply_cdef;
+    uint8_t uses_lr, apply_fg, kind, pad0;
+    // per-plane extents: padded (mi grid), LR input, cropped output
+    uint32_t pad_rows[3], pad_cols[3];
+    uint32_t lr_rows[3], lr_cols[3];
+    uint32_t out_rows[3], out_cols[3];
+
+    // ---- deblocking ----
+    uint8_t lf_level[4];
+    uint8_t lf_sharp, lf_delta_enabled, lf_delta_multi, pad1;
+    int8_t lf_ref_deltas[8];
+    int8_t lf_mode_deltas[2];
+    uint8_t pad2[2];
+    uint8_t seg_active[8][8];
+    int16_t seg_data[8][8];
+    const uint32_t *lf_mi;      // packed per-mi words (OXAV1_MI_* layout)
+    const int8_t *delta_lf;     // 4 per mi or null
+    const uint8_t *tx[3];       // per-plane loop-filter tx grid
+    uint32_t tx_cols[3];
+
+    // ---- CDEF ----
+    oxav1_cdef_params cdef;
+    const int8_t *cdef_idx;     // per 64x64 unit
+    uint32_t cdef_ucols, cdef_urows;
+    uint32_t cdef_ext_rows[3], cdef_ext_cols[3];  // extent the 8x8 walk writes
+
+    // ---- superres (per plane, computed on the host in 64-bit) ----
+    int32_t sr_step[3], sr_init[3], sr_maxx[3];
+    uint32_t sr_in_w[3];        // downscaled plane width (input columns used)
+
+    // ---- loop restoration ----
+    uint8_t lr_type[3];
+    uint8_t pad3;
+    uint32_t lr_unit_size[3], lr_unit_rows[3], lr_unit_cols[3];
+    const oxav1_lr_unit *lr_units[3];
+    uint32_t lr_tile_w[3];      // LR tile width (divides the unit size)
+    uint32_t lrc_rows[3], lrc_cols[3];  // extents of the LR "curr" buffer
+    uint8_t lr_copy_none;       // tiles without restoration copy CDEF -> out
+    uint8_t pad4[3];
+
+    // ---- film grain ----
+    oxav1_film_grain_params fg;
+    FgScratch *fgs;
+
+    // ---- buffers (pitches in samples) ----
+    const void *src[3];
+    uint32_t src_pitch[3];
+    void *deb[3];               // deblocked (padded)
+    uint32_t deb_pitch[3];
+    void *cdf[3];               // CDEF output (padded)
+    uint32_t cdf_pitch[3];
+    void *up_cdef[3];           // LR "cdef" input (upscaled or == cdf)
+    uint32_t up_cdef_pitch[3];
+    void *up_deb[3];            // LR "curr" input (upscaled or == deb)
+    uint32_t up_deb_pitch[3];
+    void *ref_out[3];           // cropped pre-grain frame
+    uint32_t ref_pitch[3];
+    void *out[3];               // film-grain output
+    uint32_t out_pitch[3];
+
+    // ---- work items per stage ----
+    // stage s, plane p: items [item_base[s][p], item_base[s][p+1]) of this frame;
+    // for superres planes 3..5 are the deblocked set.
+    uint32_t item_base[5][7];
+    uint32_t lf_tiles_x[3];
+    uint32_t lr_tiles_x[3];
+    uint32_t fg_tiles_x[3];
+    uint32_t sr_rows_per_item;
+    uint8_t lf_plane_on[3];     // plane filtered (loop_filter.rs:276-280)
+    uint8_t sr_deb_on;          // upscale the deblocked set too
+};
+
+enum Stage : int { kStageLf = 0, kStageCdef = 1, kStageSr = 2, kStageLr = 3, kStageFg = 4 };
+
+__device__ __forceinline__ int32_t clip3(int32_t a, int32_t b, int32_t x) {
+    return x < a ? a : (x > b ? b : x);
+}
+__device__ __forceinline__ int32_t round2(int32_t x, uint32_t n) {
+    return n == 0 ? x : (x + (1 << (n - 1))) >> n;
+}
+__device__ __forceinline__ uint32_t floor_log2(uint32_t x) {
+    return x == 0 ? 0u : 31u - (uint32_t)__clz((int)x);
+}
+
+template <typename T>
+__device__ __forceinline__ T *plane_ptr(void *p) {
+    return static_cast<T *>(p);
+}
+template <typename T>
+__device__ __forceinline__ const T *plane_ptr(const void *p) {
+    return static_cast<const T *>(p);
+}
+
+}  // namespace oxav1

@@ -1,0 +1,899 @@
+// Kernels, device job tables and the C ABI of the gfx950 post-filter chain.
+//
+// Stage kernels take a table of FrameJobs.  Each launch processes work items
+// (frame, plane, tile): plain launches cover one frame with one workgroup per
+// item; batched launches are persistent (about two workgroups per CU) and
+// pull items from a device counter, so frames of any size, bit depth and
+// subsampling share one launch per stage.  The batched stage sequence can be
+// captured once in a hipGraph and replayed.
+//
+// The drop-in entry points (oxav1_pf_loop_filter_frame etc.) run the same
+// tile functions instantiated for int32 samples on staged copies of the
+// caller's PlaneBuffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/oxav1_pf.h"
+#include "av1_tables.h"
+#include "cdef.hpp"
+#include "common.hpp"
+#include "fg.hpp"
+#include "lf.hpp"
+#include "lr.hpp"
+#include "sr.hpp"
+
+namespace oxav1 {
+
+// ===================================================================== //
+// Device side                                                           //
+// ===================================================================== //
+
+constexpr int kThreads = 256;
+
+// (frame, local item) for an item of a stage
+struct ItemRef {
+    uint32_t frame, local;
+};
+__device__ __forceinline__ ItemRef find_item(const uint32_t *prefix, uint32_t njobs, uint32_t item) {
+    if (!prefix) return {0u, item};
+    uint32_t lo = 0, hi = njobs;  // prefix[lo] <= item < prefix[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return {lo, item - prefix[lo]};
+}
+__device__ __forceinline__ uint32_t plane_of(const uint32_t *base, uint32_t nplanes, uint32_t local) {
+    uint32_t p = 0;
+    while (p + 1 < nplanes && local >= base[p + 1]) p++;
+    return p;
+}
+
+// --- stage bodies (templated on the sample type) ---
+struct LfShared {
+    uint16_t dec0[kLfCellRows * kLfEdges];
+    uint16_t dec1[kLfEdges * (kLfTile / 4)];
+};
+
+template <typename T>
+__device__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *region, LfShared &sh) {
+    const uint32_t p = plane_of(J.item_base[kStageLf], 3, local);
+    const uint32_t t = local - J.item_base[kStageLf][p];
+    const uint32_t ty = t / J.lf_tiles_x[p], tx = t - ty * J.lf_tiles_x[p];
+    lf_tile<T>(J, p, ty, tx, plane_ptr<T>(J.src[p]), J.src_pitch[p], plane_ptr<T>(J.deb[p]), J.deb_pitch[p], region,
+               sh.dec0, sh.dec1);
+}
+
+template <typename T>
+__device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk) {
+    const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
+    cdef_unit<T>(J, ur, uc, tile, blk);
+}
+
+template <typename T>
+__device__ void sr_item(const FrameJob &J, uint32_t local) {
+    const uint32_t p6 = plane_of(J.item_base[kStageSr], 6, local);
+    const uint32_t chunk = local - J.item_base[kStageSr][p6];
+    const uint32_t p = p6 % 3;
+    const bool deb = p6 >= 3;
+    const int32_t y0 = (int32_t)(chunk * J.sr_rows_per_item);
+    const int32_t y1 = min(y0 + (int32_t)J.sr_rows_per_item, (int32_t)J.out_rows[p]);
+    const T *in = plane_ptr<T>(deb ? (const void *)J.deb[p] : (const void *)J.cdf[p]);
+    const uint32_t ip = deb ? J.deb_pitch[p] : J.cdf_pitch[p];
+    T *out = plane_ptr<T>(deb ? J.up_deb[p] : J.up_cdef[p]);
+    const uint32_t op = deb ? J.up_deb_pitch[p] : J.up_cdef_pitch[p];
+    sr_rows<T>(in, ip, (int32_t)J.pad_cols[p], out, op, (int32_t)J.out_cols[p], y0, y1, J.sr_step[p], J.sr_init[p],
+               J.sr_maxx[p], J.bd);
+}
+
+template <typename T>
+__device__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
+    const uint32_t p = plane_of(J.item_base[kStageLr], 3, local);
+    const uint32_t t = local - J.item_base[kStageLr][p];
+    const uint32_t stripe = t / J.lr_tiles_x[p], tile = t - stripe * J.lr_tiles_x[p];
+    lr_tile<T>(J, p, stripe, tile, S, plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]);
+}
+
+template <typename T>
+__device__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
+    const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
+    const uint32_t t = local - J.item_base[kStageFg][p];
+    const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
+    for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = J.fgs->lut[i >> 8][i & 255];
+    __syncthreads();
+    const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
+    const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
+    fg_blend_rows<T>(J, p, y0, y1, x0, x1, lut);
+}
+
+// --- kernels: KIND 0 = engine (u8/u16 by FrameJob::kind), 1 = int32 ---
+// counter == nullptr: one workgroup per item (blockIdx.x); else persistent.
+#define OXAV1_ITEM_LOOP(BODY)                                                     \
+    __shared__ uint32_t s_item;                                                   \
+    for (uint32_t iter = 0;; iter++) {                                            \
+        uint32_t item;                                                            \
+        if (counter) {                                                            \
+            if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);                \
+            __syncthreads();                                                      \
+            item = s_item;                                                        \
+            __syncthreads();                                                      \
+        } else {                                                                  \
+            if (iter > 0) break;                                                  \
+            item = blockIdx.x;                                                    \
+        }                                                                         \
+        if (item >= total) break;                                                 \
+        const ItemRef ref = find_item(prefix, njobs, item);                       \
+        const FrameJob &J = jobs[ref.frame];                                      \
+        BODY;                                                                     \
+        __syncthreads();                                                          \
+    }
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_lf(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                 uint32_t total, uint32_t *counter) {
+    using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
+    __shared__ L region[kLfRegion * kLfRegion];
+    __shared__ LfShared sh;
+    OXAV1_ITEM_LOOP({
+        if constexpr (KIND == 1) {
+            lf_item<int32_t>(J, ref.local, region, sh);
+        } else {
+            if (J.kind == kU8)
+                lf_item<uint8_t>(J, ref.local, region, sh);
+            else
+                lf_item<uint16_t>(J, ref.local, region, sh);
+        }
+    })
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                   uint32_t total, uint32_t *counter) {
+    using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
+    __shared__ L tile[kCdefLw * kCdefLw];
+    __shared__ CdefBlk blk[64];
+    OXAV1_ITEM_LOOP({
+        if constexpr (KIND == 1) {
+            cdef_item<int32_t>(J, ref.local, tile, blk);
+        } else {
+            if (J.kind == kU8)
+                cdef_item<uint8_t>(J, ref.local, tile, blk);
+            else
+                cdef_item<uint16_t>(J, ref.local, tile, blk);
+        }
+    })
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_sr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                 uint32_t total, uint32_t *counter) {
+    OXAV1_ITEM_LOOP({
+        if constexpr (KIND == 1) {
+            sr_item<int32_t>(J, ref.local);
+        } else {
+            if (J.kind == kU8)
+                sr_item<uint8_t>(J, ref.local);
+            else
+                sr_item<uint16_t>(J, ref.local);
+        }
+    })
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_lr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                 uint32_t total, uint32_t *counter) {
+    using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
+    __shared__ LrSmem<L> S;
+    OXAV1_ITEM_LOOP({
+        if constexpr (KIND == 1) {
+            lr_item<int32_t>(J, ref.local, S);
+        } else {
+            if (J.kind == kU8)
+                lr_item<uint8_t>(J, ref.local, S);
+            else
+                lr_item<uint16_t>(J, ref.local, S);
+        }
+    })
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_fg(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                 uint32_t total, uint32_t *counter) {
+    __shared__ uint8_t lut[768];
+    OXAV1_ITEM_LOOP({
+        if constexpr (KIND == 1) {
+            fg_item<int32_t>(J, ref.local, lut);
+        } else {
+            if (J.kind == kU8)
+                fg_item<uint8_t>(J, ref.local, lut);
+            else
+                fg_item<uint16_t>(J, ref.local, lut);
+        }
+    })
+}
+
+// One workgroup per frame; frames without film grain return at once.
+__global__ __launch_bounds__(kThreads) void k_fg_prep(const FrameJob *jobs, uint32_t njobs) {
+    __shared__ int16_t sm[3 * kGrainH * kGrainW];
+    if (blockIdx.x >= njobs) return;
+    const FrameJob &J = jobs[blockIdx.x];
+    if (!J.apply_fg) return;
+    fg_prep_frame(J, sm);
+}
+
+// Zero the batched work counters (a kernel, so the reset is graph-safe).
+__global__ void k_reset(uint32_t *counters, uint32_t n) {
+    if (threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+
+// ===================================================================== //
+// Host side                                                             //
+// ===================================================================== //
+
+thread_local std::string g_last_hip;
+
+static int hip_fail(hipError_t e, const char *what) {
+    g_last_hip = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? OXAV1_ERR_OUT_OF_MEMORY : OXAV1_ERR_HIP;
+}
+#define HIP_TRY(expr)                                    \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+static std::mutex g_init_mu;
+static bool g_init_done[64];
+
+// Constant tables per device, once.
+static int ensure_device_init() {
+    int dev = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return OXAV1_ERR_NO_DEVICE;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (dev < 64 && g_init_done[dev]) return OXAV1_OK;
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_subsampled), kSubsampledSize, sizeof(kSubsampledSize)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_upscale_filter), kUpscaleFilter, sizeof(kUpscaleFilter)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_params), kSgrParams, sizeof(kSgrParams)));
+    uint16_t a2[256] = {0};
+    for (int z = 1; z < 255; z++) a2[z] = (uint16_t)(((z << 8) + z / 2) / (z + 1));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_a2), a2, sizeof(a2)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), kGaussianSequence, sizeof(kGaussianSequence)));
+    // LFSR step matrix (film_grain.rs:117-130) and its powers 2^b.
+    uint16_t m[16][16];
+    for (int i = 0; i < 16; i++) {
+        uint32_t col = i >= 1 ? (1u << (i - 1)) : 0u;
+        if (i == 0 || i == 1 || i == 3 || i == 12) col |= 1u << 15;
+        m[0][i] = (uint16_t)col;
+    }
+    for (int b = 1; b < 16; b++)
+        for (int i = 0; i < 16; i++) {
+            // column i of M^(2^b) = M^(2^(b-1)) applied to column i of M^(2^(b-1))
+            uint32_t v = m[b - 1][i], r = 0;
+            for (int k = 0; k < 16; k++)
+                if ((v >> k) & 1u) r ^= m[b - 1][k];
+            m[b][i] = (uint16_t)r;
+        }
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_lfsr_jump), m, sizeof(m)));
+    if (dev < 64) g_init_done[dev] = true;
+    return OXAV1_OK;
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n) return OXAV1_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+        n = bytes;
+        return OXAV1_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+static inline uint32_t rnd2(uint32_t x, uint32_t n) { return n == 0 ? x : (x + (1u << (n - 1))) >> n; }
+static inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// superres constants (superres.rs:363-382), 64-bit with truncating division.
+static void sr_consts(uint32_t dw, uint32_t uw, uint32_t mi_cols, uint32_t sx, int32_t *step, int32_t *init,
+                      int32_t *maxx) {
+    const int64_t stepx_num = ((int64_t)dw << 14) + (int64_t)(uw / 2);
+    const int32_t st = (int32_t)(stepx_num / (int64_t)uw);
+    const int32_t err = (int32_t)((int64_t)uw * (int64_t)st - ((int64_t)dw << 14));
+    const int64_t diff = (int64_t)uw - (int64_t)dw;
+    const int64_t num = -(diff << 13) + (int64_t)(uw / 2);
+    const int32_t division = (int32_t)(num / (int64_t)uw);
+    int32_t isx = division + (1 << 7) - err / 2;
+    isx &= (1 << 14) - 1;
+    *step = st;
+    *init = isx;
+    *maxx = (int32_t)(mi_cols >> sx) * 4 - 1;
+}
+
+// LR tile width for a unit size: must divide it; stripes never straddle units.
+static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw) {
+    if (us < 32 || us % 32 != 0 || us < (64u >> sy)) return OXAV1_ERR_UNSUPPORTED;
+    *tw = (us % 64 == 0) ? 64u : 32u;
+    return OXAV1_OK;
+}
+
+static uint32_t lr_stripe_count(uint32_t pey, uint32_t sy) { return ((pey << sy) + 8) / 64 + 1; }
+
+}  // namespace oxav1
+
+using namespace oxav1;
+
+// --------------------------------------------------------------------- //
+// Engine                                                                //
+// --------------------------------------------------------------------- //
+
+struct FrameSlot {
+    DevBuf deb[3], cdf[3], upc[3], upd[3], fgs, offs;
+    void release() {
+        for (int p = 0; p < 3; p++) {
+            deb[p].release();
+            cdf[p].release();
+            upc[p].release();
+            upd[p].release();
+        }
+        fgs.release();
+        offs.release();
+    }
+};
+
+struct oxav1_pf_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t max_frames = 0;
+    int num_cus = 0;
+    std::vector<FrameSlot> slots;
+    std::vector<FrameJob> jobs_h;
+    DevBuf jobs_d, prefix_d, counters_d;
+    std::vector<uint32_t> prefix_h;  // 5 x (n+1)
+    uint32_t n = 0;
+    uint32_t totals[5] = {0, 0, 0, 0, 0};
+    bool any_fg = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    double stage_sum[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t stage_cnt[6] = {0, 0, 0, 0, 0, 0};
+    int persistent_blocks = 0;
+};
+
+static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
+
+static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_frame_dev &v, FrameSlot &slot,
+                     FrameJob &J) {
+    std::memset(&J, 0, sizeof J);
+    if (d.bit_depth != 8 && d.bit_depth != 10 && d.bit_depth != 12) return OXAV1_ERR_INVALID_ARG;
+    if (d.num_planes != 1 && d.num_planes != 3) return OXAV1_ERR_INVALID_ARG;
+    if (d.mi_rows == 0 || d.mi_cols == 0 || d.frame_width == 0 || d.frame_height == 0) return OXAV1_ERR_INVALID_ARG;
+    const uint32_t np = d.num_planes, sx = d.subsampling_x ? 1 : 0, sy = d.subsampling_y ? 1 : 0;
+    const size_t B = d.bit_depth == 8 ? 1 : 2;
+    J.fw = d.frame_width;
+    J.fh = d.frame_height;
+    J.uw = d.upscaled_width ? d.upscaled_width : d.frame_width;
+    J.mi_rows = d.mi_rows;
+    J.mi_cols = d.mi_cols;
+    J.bd = d.bit_depth;
+    J.sx = (uint8_t)sx;
+    J.sy = (uint8_t)sy;
+    J.np = (uint8_t)np;
+    J.mc = d.matrix_coefficients;
+    J.kind = d.bit_depth == 8 ? kU8 : kU16;
+    J.use_sr = d.use_superres && J.uw > J.fw;
+    J.apply_lf = d.apply_lf;
+    J.apply_cdef = d.apply_cdef;
+    J.uses_lr = d.lr.uses_lr;
+    J.apply_fg = d.fg.apply_grain;
+    for (uint32_t p = 0; p < 3; p++) {
+        const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+        J.pad_rows[p] = p < np ? (d.mi_rows * 4) >> psy : 0;
+        J.pad_cols[p] = p < np ? (d.mi_cols * 4) >> psx : 0;
+        J.out_rows[p] = p < np ? rnd2(J.fh, psy) : 0;
+        J.out_cols[p] = p < np ? rnd2(J.uw, psx) : 0;
+        if (p < np && (J.out_rows[p] > J.pad_rows[p] || (!J.use_sr && J.out_cols[p] > J.pad_cols[p])))
+            return OXAV1_ERR_INVALID_ARG;  // frame_driver.rs:1030-1032
+    }
+    // ---- deblocking ----
+    std::memcpy(J.lf_level, d.lf.loop_filter_level, 4);
+    J.lf_sharp = d.lf.loop_filter_sharpness;
+    J.lf_delta_enabled = d.lf.loop_filter_delta_enabled;
+    J.lf_delta_multi = d.lf.delta_lf_multi;
+    std::memcpy(J.lf_ref_deltas, d.lf.loop_filter_ref_deltas, 8);
+    std::memcpy(J.lf_mode_deltas, d.lf.loop_filter_mode_deltas, 2);
+    std::memcpy(J.seg_active, d.seg_feature_active, sizeof J.seg_active);
+    std::memcpy(J.seg_data, d.seg_feature_data, sizeof J.seg_data);
+    J.lf_mi = v.lf_mi;
+    J.delta_lf = v.delta_lf;
+    for (uint32_t p = 0; p < 3; p++) {
+        J.tx[p] = v.lf_tx_size[p];
+        J.tx_cols[p] = (d.mi_cols + (p ? sx : 0)) >> (p ? sx : 0);
+        J.lf_plane_on[p] = J.apply_lf && p < np && (p == 0 || d.lf.loop_filter_level[1 + p] != 0);
+    }
+    if (J.apply_lf && (!v.lf_mi || !v.lf_tx_size[0] || (np > 1 && (!v.lf_tx_size[1] || !v.lf_tx_size[2]))))
+        return OXAV1_ERR_INVALID_ARG;
+    // ---- CDEF ----
+    J.cdef = d.cdef;
+    J.cdef_idx = v.cdef_idx;
+    J.cdef_ucols = ceil_div(d.mi_cols, 16);
+    J.cdef_urows = ceil_div(d.mi_rows, 16);
+    if (J.apply_cdef && (!v.cdef_idx || !v.lf_mi)) return OXAV1_ERR_INVALID_ARG;
+    for (uint32_t p = 0; p < np; p++) {
+        const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+        J.cdef_ext_rows[p] = std::min(J.pad_rows[p], (ceil_div(d.mi_rows, 2) * 8) >> psy);
+        J.cdef_ext_cols[p] = std::min(J.pad_cols[p], (ceil_div(d.mi_cols, 2) * 8) >> psx);
+    }
+    // ---- buffers: src -> deb -> cdf -> (SR) -> LR -> ref_out -> FG -> out ----
+    for (uint32_t p = 0; p < np; p++) {
+        if (!v.src[p] || !v.ref_out[p]) return OXAV1_ERR_INVALID_ARG;
+        J.src[p] = v.src[p];
+        J.src_pitch[p] = v.src_pitch[p] ? v.src_pitch[p] : J.pad_cols[p];
+        const size_t padn = (size_t)J.pad_rows[p] * J.pad_cols[p] * B;
+        if (J.lf_plane_on[p]) {
+            int st = slot.deb[p].ensure(padn);
+            if (st) return st;
+            J.deb[p] = slot.deb[p].p;
+            J.deb_pitch[p] = J.pad_cols[p];
+        } else {
+            J.deb[p] = const_cast<void *>(v.src[p]);
+            J.deb_pitch[p] = J.src_pitch[p];
+        }
+        if (J.apply_cdef) {
+            int st = slot.cdf[p].ensure(padn);
+            if (st) return st;
+            J.cdf[p] = slot.cdf[p].p;
+            J.cdf_pitch[p] = J.pad_cols[p];
+        } else {
+            J.cdf[p] = J.deb[p];
+            J.cdf_pitch[p] = J.deb_pitch[p];
+        }
+    }
+    // ---- superres ----
+    J.sr_rows_per_item = 8;
+    if (J.use_sr) {
+        for (uint32_t p = 0; p < np; p++) {
+            const uint32_t psx = p ? sx : 0;
+            const uint32_t dw = rnd2(J.fw, psx), uw = rnd2(J.uw, psx);
+            if (uw <= dw) return OXAV1_ERR_UPSCALED_NOT_LARGER;
+            sr_consts(dw, uw, d.mi_cols, psx, &J.sr_step[p], &J.sr_init[p], &J.sr_maxx[p]);
+            J.sr_in_w[p] = dw;
+            const size_t outn = (size_t)J.out_rows[p] * J.out_cols[p] * B;
+            int st = slot.upc[p].ensure(outn);
+            if (st) return st;
+            J.up_cdef[p] = slot.upc[p].p;
+            J.up_cdef_pitch[p] = J.out_cols[p];
+            if (J.uses_lr) {
+                st = slot.upd[p].ensure(outn);
+                if (st) return st;
+                J.up_deb[p] = slot.upd[p].p;
+                J.up_deb_pitch[p] = J.out_cols[p];
+            }
+            J.lr_rows[p] = J.lrc_rows[p] = J.out_rows[p];
+            J.lr_cols[p] = J.lrc_cols[p] = J.out_cols[p];
+        }
+        J.sr_deb_on = J.uses_lr;
+    } else {
+        for (uint32_t p = 0; p < np; p++) {
+            J.up_cdef[p] = J.cdf[p];
+            J.up_cdef_pitch[p] = J.cdf_pitch[p];
+            J.up_deb[p] = J.deb[p];
+            J.up_deb_pitch[p] = J.deb_pitch[p];
+            J.lr_rows[p] = J.lrc_rows[p] = J.pad_rows[p];
+            J.lr_cols[p] = J.lrc_cols[p] = J.pad_cols[p];
+        }
+    }
+    // ---- loop restoration ----
+    J.lr_copy_none = 1;
+    for (uint32_t p = 0; p < np; p++) {
+        const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+        J.lr_type[p] = J.uses_lr ? d.lr.frame_restoration_type[p] : OXAV1_RESTORE_NONE;
+        J.lr_units[p] = v.lr_units[p];
+        J.lr_tile_w[p] = 64;
+        if (J.lr_type[p] != OXAV1_RESTORE_NONE) {
+            const uint32_t us = d.lr.loop_restoration_size[p];
+            int st = lr_tile_width(us, psy, &J.lr_tile_w[p]);
+            if (st) return st;
+            J.lr_unit_size[p] = us;
+            J.lr_unit_rows[p] = oxav1_pf_count_units_in_frame(us, rnd2(J.fh, psy));
+            J.lr_unit_cols[p] = oxav1_pf_count_units_in_frame(us, rnd2(J.uw, psx));
+        }
+        J.ref_out[p] = v.ref_out[p];
+        J.ref_pitch[p] = v.ref_pitch[p] ? v.ref_pitch[p] : J.out_cols[p];
+    }
+    // ---- film grain ----
+    for (uint32_t p = 0; p < np; p++) {
+        J.out[p] = v.out[p] ? v.out[p] : v.ref_out[p];
+        J.out_pitch[p] = v.out[p] ? (v.out_pitch[p] ? v.out_pitch[p] : J.out_cols[p]) : J.ref_pitch[p];
+        if (J.apply_fg && J.out[p] == J.ref_out[p]) return OXAV1_ERR_ALIAS;
+    }
+    J.fg = d.fg;
+    const bool fg_stage = J.apply_fg || (J.out[0] != J.ref_out[0]);
+    if (J.apply_fg) {
+        int st = slot.fgs.ensure(sizeof(FgScratch));
+        if (st) return st;
+        const uint32_t nst = ceil_div(ceil_div(J.fh, 2), 16), nbl = ceil_div(ceil_div(J.uw, 2), 16);
+        st = slot.offs.ensure((size_t)nst * nbl);
+        if (st) return st;
+        J.fgs = (FgScratch *)slot.fgs.p;
+        FgScratch hs;
+        std::memset(&hs, 0, sizeof hs);
+        hs.nstripes = nst;
+        hs.nblocks = nbl;
+        hs.offsets = (uint8_t *)slot.offs.p;
+        HIP_TRY(hipMemcpy(slot.fgs.p, &hs, sizeof hs, hipMemcpyHostToDevice));
+    }
+    // ---- item tables ----
+    uint32_t acc = 0;
+    for (uint32_t p = 0; p < 3; p++) {
+        J.item_base[kStageLf][p] = acc;
+        J.lf_tiles_x[p] = ceil_div(J.pad_cols[p], kLfTile);
+        if (p < np && J.lf_plane_on[p]) acc += ceil_div(J.pad_rows[p], kLfTile) * J.lf_tiles_x[p];
+    }
+    J.item_base[kStageLf][3] = acc;
+    J.item_base[kStageCdef][0] = 0;
+    J.item_base[kStageCdef][1] = J.apply_cdef ? J.cdef_urows * J.cdef_ucols : 0;
+    acc = 0;
+    for (uint32_t p6 = 0; p6 < 6; p6++) {
+        J.item_base[kStageSr][p6] = acc;
+        const uint32_t p = p6 % 3;
+        const bool on = J.use_sr && p < np && (p6 < 3 || J.sr_deb_on);
+        if (on) acc += ceil_div(J.out_rows[p], J.sr_rows_per_item);
+    }
+    J.item_base[kStageSr][6] = acc;
+    acc = 0;
+    for (uint32_t p = 0; p < 3; p++) {
+        J.item_base[kStageLr][p] = acc;
+        if (p < np) {
+            const uint32_t psy = p ? sy : 0;
+            J.lr_tiles_x[p] = ceil_div(J.out_cols[p], J.lr_tile_w[p]);
+            acc += lr_stripe_count(J.out_rows[p] - 1, psy) * J.lr_tiles_x[p];
+        }
+    }
+    J.item_base[kStageLr][3] = acc;
+    acc = 0;
+    for (uint32_t p = 0; p < 3; p++) {
+        J.item_base[kStageFg][p] = acc;
+        J.fg_tiles_x[p] = ceil_div(J.out_cols[p], 64);
+        if (p < np && fg_stage) acc += ceil_div(J.out_rows[p], 64) * J.fg_tiles_x[p];
+    }
+    J.item_base[kStageFg][3] = acc;
+    (void)e;
+    return OXAV1_OK;
+}
+
+static uint32_t stage_items(const FrameJob &J, int s) {
+    switch (s) {
+    case kStageLf: return J.item_base[kStageLf][3];
+    case kStageCdef: return J.item_base[kStageCdef][1];
+    case kStageSr: return J.item_base[kStageSr][6];
+    case kStageLr: return J.item_base[kStageLr][3];
+    default: return J.item_base[kStageFg][3];
+    }
+}
+
+extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, uint32_t max_frames) {
+    if (!out) return OXAV1_ERR_INVALID_ARG;
+    *out = nullptr;
+    int st = ensure_device_init();
+    if (st) return st;
+    auto *e = new (std::nothrow) oxav1_pf_engine();
+    if (!e) return OXAV1_ERR_OUT_OF_MEMORY;
+    (void)hipGetDevice(&e->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) e->num_cus = prop.multiProcessorCount;
+    if (e->num_cus <= 0) e->num_cus = 256;
+    e->persistent_blocks = e->num_cus * 2;
+    e->max_frames = max_frames ? max_frames : 1;
+    if (hip_stream) {
+        e->stream = (hipStream_t)hip_stream;
+    } else {
+        if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete e;
+            return OXAV1_ERR_HIP;
+        }
+        e->own_stream = true;
+    }
+    for (auto &ev : e->ev)
+        if (hipEventCreate(&ev) != hipSuccess) {
+            delete e;
+            return OXAV1_ERR_HIP;
+        }
+    st = e->counters_d.ensure(8 * sizeof(uint32_t));
+    if (st) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_destroy(oxav1_pf_engine *e) {
+    if (!e) return OXAV1_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (auto &s : e->slots) s.release();
+    e->jobs_d.release();
+    e->prefix_d.release();
+    e->counters_d.release();
+    for (auto &ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->own_stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_desc *desc, const oxav1_frame_dev *dev,
+                                       uint32_t n) {
+    if (!e || !desc || !dev || n == 0) return OXAV1_ERR_INVALID_ARG;
+    if (n > e->max_frames) return OXAV1_ERR_ADMISSION_REFUSED;
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->slots.size() < n) e->slots.resize(n);
+    e->jobs_h.resize(n);
+    e->any_fg = false;
+    for (uint32_t f = 0; f < n; f++) {
+        int st = build_job(e, desc[f], dev[f], e->slots[f], e->jobs_h[f]);
+        if (st) return st;
+        e->any_fg |= e->jobs_h[f].apply_fg != 0;
+    }
+    e->n = n;
+    e->prefix_h.assign(5 * (n + 1), 0);
+    for (int s = 0; s < 5; s++) {
+        uint32_t acc = 0;
+        for (uint32_t f = 0; f < n; f++) {
+            e->prefix_h[s * (n + 1) + f] = acc;
+            acc += stage_items(e->jobs_h[f], s);
+        }
+        e->prefix_h[s * (n + 1) + n] = acc;
+        e->totals[s] = acc;
+    }
+    int st = e->jobs_d.ensure(sizeof(FrameJob) * n);
+    if (st) return st;
+    st = e->prefix_d.ensure(sizeof(uint32_t) * e->prefix_h.size());
+    if (st) return st;
+    HIP_TRY(hipMemcpy(e->jobs_d.p, e->jobs_h.data(), sizeof(FrameJob) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->prefix_d.p, e->prefix_h.data(), sizeof(uint32_t) * e->prefix_h.size(),
+                      hipMemcpyHostToDevice));
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    return OXAV1_OK;
+}
+
+using StageKernel = void (*)(const FrameJob *, const uint32_t *, uint32_t, uint32_t, uint32_t *);
+static StageKernel kEngineKernels[5] = {k_lf<0>, k_cdef<0>, k_sr<0>, k_lr<0>, k_fg<0>};
+
+// Plain path: per frame, one launch per stage sized to the frame.
+static int enqueue_plain(oxav1_pf_engine *e, bool timed) {
+    const FrameJob *jd = (const FrameJob *)e->jobs_d.p;
+    for (uint32_t f = 0; f < e->n; f++) {
+        const FrameJob &J = e->jobs_h[f];
+        if (J.apply_fg) {
+            hipLaunchKernelGGL(k_fg_prep, dim3(1), dim3(kThreads), 0, e->stream, jd + f, 1u);
+            HIP_TRY(hipGetLastError());
+        }
+        if (timed && f == 0) HIP_TRY(hipEventRecord(e->ev[0], e->stream));
+        for (int s = 0; s < 5; s++) {
+            const uint32_t items = stage_items(J, s);
+            if (items) {
+                hipLaunchKernelGGL(kEngineKernels[s], dim3(items), dim3(kThreads), 0, e->stream, jd + f,
+                                   (const uint32_t *)nullptr, 1u, items, (uint32_t *)nullptr);
+                HIP_TRY(hipGetLastError());
+            }
+            if (timed && f == 0) HIP_TRY(hipEventRecord(e->ev[s + 1], e->stream));
+        }
+    }
+    return OXAV1_OK;
+}
+
+// Batched path: one persistent launch per stage over every frame.
+static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
+    const FrameJob *jd = (const FrameJob *)e->jobs_d.p;
+    uint32_t *ctr = (uint32_t *)e->counters_d.p;
+    const uint32_t *pre = (const uint32_t *)e->prefix_d.p;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, e->stream, ctr, 8u);
+    HIP_TRY(hipGetLastError());
+    if (e->any_fg) {
+        hipLaunchKernelGGL(k_fg_prep, dim3(e->n), dim3(kThreads), 0, e->stream, jd, e->n);
+        HIP_TRY(hipGetLastError());
+    }
+    if (timed) HIP_TRY(hipEventRecord(e->ev[0], e->stream));
+    for (int s = 0; s < 5; s++) {
+        const uint32_t total = e->totals[s];
+        if (total) {
+            const uint32_t grid = std::min<uint32_t>(total, (uint32_t)e->persistent_blocks);
+            hipLaunchKernelGGL(kEngineKernels[s], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
+                               e->n, total, ctr + s);
+            HIP_TRY(hipGetLastError());
+        }
+        if (timed) HIP_TRY(hipEventRecord(e->ev[s + 1], e->stream));
+    }
+    return OXAV1_OK;
+}
+
+static int collect_timing(oxav1_pf_engine *e) {
+    HIP_TRY(hipEventSynchronize(e->ev[5]));
+    for (int s = 0; s < 5; s++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev[s], e->ev[s + 1]));
+        e->stage_sum[s] += ms;
+        e->stage_cnt[s]++;
+    }
+    return OXAV1_OK;
+}
+
+// mode 0 = plain, 1 = batched, 2 = batched replayed from a hipGraph
+extern "C" int oxav1_pf_engine_run(oxav1_pf_engine *e, int mode) {
+    if (!e || e->n == 0) return OXAV1_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(e->device));
+    if (mode == 0) {
+        int st = enqueue_plain(e, e->timing);
+        if (st) return st;
+        return e->timing ? collect_timing(e) : OXAV1_OK;
+    }
+    if (mode == 1 || e->timing) {
+        int st = enqueue_batch(e, e->timing);
+        if (st) return st;
+        return e->timing ? collect_timing(e) : OXAV1_OK;
+    }
+    if (mode != 2) return OXAV1_ERR_INVALID_ARG;
+    if (!e->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_batch(e, false);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+        e->graph = g;
+        HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_run_frame(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
+                                         const oxav1_frame_dev *dev) {
+    int st = oxav1_pf_engine_prepare(e, desc, dev, 1);
+    return st ? st : oxav1_pf_engine_run(e, 0);
+}
+
+extern "C" int oxav1_pf_engine_run_batch(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
+                                         const oxav1_frame_dev *dev, uint32_t n, int use_graph) {
+    int st = oxav1_pf_engine_prepare(e, desc, dev, n);
+    return st ? st : oxav1_pf_engine_run(e, use_graph ? 2 : 1);
+}
+
+extern "C" int oxav1_pf_engine_sync(oxav1_pf_engine *e) {
+    if (!e) return OXAV1_ERR_INVALID_ARG;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_set_timing(oxav1_pf_engine *e, int enable) {
+    if (!e) return OXAV1_ERR_INVALID_ARG;
+    e->timing = enable != 0;
+    for (int s = 0; s < 6; s++) {
+        e->stage_sum[s] = 0;
+        e->stage_cnt[s] = 0;
+    }
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_stage_ms(oxav1_pf_engine *e, const char *stage, double *avg_ms, uint32_t *count) {
+    if (!e || !stage || !avg_ms) return OXAV1_ERR_INVALID_ARG;
+    for (int s = 0; s < 5; s++)
+        if (!std::strcmp(stage, kStageNames[s])) {
+            *avg_ms = e->stage_cnt[s] ? e->stage_sum[s] / e->stage_cnt[s] : 0.0;
+            if (count) *count = e->stage_cnt[s];
+            return OXAV1_OK;
+        }
+    return OXAV1_ERR_INVALID_ARG;
+}
+
+extern "C" void *oxav1_pf_engine_stream(oxav1_pf_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+// --------------------------------------------------------------------- //
+// Library basics                                                        //
+// --------------------------------------------------------------------- //
+
+extern "C" int oxav1_pf_init(void) { return ensure_device_init(); }
+
+extern "C" const char *oxav1_pf_strerror(int status) {
+    switch (status) {
+    case OXAV1_OK: return "ok";
+    case OXAV1_ERR_UPSCALED_NOT_LARGER: return "superres: upscaled width not larger than frame width";
+    case OXAV1_ERR_PLANE_SHAPE_MISMATCH: return "plane shape mismatch";
+    case OXAV1_ERR_PLANE_COUNT_MISMATCH: return "plane count mismatch";
+    case OXAV1_ERR_INVALID_ARG: return "invalid argument";
+    case OXAV1_ERR_ALIAS: return "input and output buffers alias";
+    case OXAV1_ERR_HIP: return "HIP runtime error";
+    case OXAV1_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case OXAV1_ERR_ADMISSION_REFUSED: return "admission refused";
+    case OXAV1_ERR_UNSUPPORTED: return "unsupported parameter combination";
+    case OXAV1_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+extern "C" const char *oxav1_pf_last_hip_error(void) { return g_last_hip.c_str(); }
+
+extern "C" uint32_t oxav1_pf_count_units_in_frame(uint32_t unit_size, uint32_t frame_size) {
+    if (unit_size == 0) return 1;
+    const uint32_t v = (frame_size + (unit_size >> 1)) / unit_size;
+    return v > 1 ? v : 1;
+}
+
+// Checksum of a constant table of the library (tests compare it with the
+// values parsed from the reference): sum of (i+1) * value.
+extern "C" int64_t oxav1_pf_table_checksum(const char *name) {
+    auto ck = [](const auto &arr) {
+        int64_t s = 0;
+        size_t i = 0;
+        for (auto v : arr) s += (int64_t)(++i) * (int64_t)v;
+        return s;
+    };
+    if (!name) return INT64_MIN;
+    std::string n(name);
+    if (n == "GAUSSIAN_SEQUENCE") return ck(kGaussianSequence);
+    if (n == "UPSCALE_FILTER") return ck(kUpscaleFilter);
+    if (n == "SGR_PARAMS") return ck(kSgrParams);
+    if (n == "CDEF_DIRECTIONS") return ck(kCdefDirections);
+    if (n == "CDEF_UV_DIR") return ck(kCdefUvDir);
+    if (n == "NUM_4X4_BLOCKS_WIDE") return ck(kNum4x4W);
+    if (n == "NUM_4X4_BLOCKS_HIGH") return ck(kNum4x4H);
+    if (n == "TX_WIDTH") return ck(kTxWidth);
+    if (n == "TX_HEIGHT") return ck(kTxHeight);
+    if (n == "SUBSAMPLED_SIZE") return ck(kSubsampledSize);
+    if (n == "MAX_TX_SIZE_RECT") return ck(kMaxTxSizeRect);
+    return INT64_MIN;
+}
+
+extern "C" size_t oxav1_pf_struct_size(const char *name) {
+    if (!name) return 0;
+    std::string n(name);
+    if (n == "oxav1_plane_i32") return sizeof(oxav1_plane_i32);
+    if (n == "oxav1_lf_ctx") return sizeof(oxav1_lf_ctx);
+    if (n == "oxav1_lf_grids") return sizeof(oxav1_lf_grids);
+    if (n == "oxav1_cdef_params") return sizeof(oxav1_cdef_params);
+    if (n == "oxav1_cdef_ctx") return sizeof(oxav1_cdef_ctx);
+    if (n == "oxav1_cdef_grids") return sizeof(oxav1_cdef_grids);
+    if (n == "oxav1_sr_ctx") return sizeof(oxav1_sr_ctx);
+    if (n == "oxav1_lr_unit") return sizeof(oxav1_lr_unit);
+    if (n == "oxav1_lr_ctx") return sizeof(oxav1_lr_ctx);
+    if (n == "oxav1_lr_grids") return sizeof(oxav1_lr_grids);
+    if (n == "oxav1_film_grain_params") return sizeof(oxav1_film_grain_params);
+    if (n == "oxav1_frame_desc") return sizeof(oxav1_frame_desc);
+    if (n == "oxav1_frame_dev") return sizeof(oxav1_frame_dev);
+    return 0;
+}
+
+#include "dropin.inc"

@@ -1,0 +1,357 @@
+// §7.18.3 film-grain synthesis on gfx950.
+//
+// Reference: film_grain.rs:100-873.  Two kernels per frame:
+//   fg_prep  (one workgroup): scaling LUTs (§7.18.3.4), the per-(stripe,
+//            32x32 block) LFSR offsets, and the grain templates (§7.18.3.3).
+//            The white-noise LFSR is linear over GF(2), so each lane jumps to
+//            its chunk of the sequence with precomputed powers of the step
+//            matrix; the causal auto-regressive filter runs as a wavefront
+//            with one lane per template row, row y trailing row y-1 by four
+//            columns (the filter reaches x+3 on earlier rows).
+//   fg_blend (whole frame): per output sample, the noise value is rebuilt
+//            from the templates and offsets, including the horizontal and
+//            vertical 27/17 (or 23/22) overlap blends of the reference's
+//            noise stripes (film_grain.rs:626-739), then blended with the
+//            scaling LUT (film_grain.rs:759-828).  Chroma reads the
+//            pre-noise luma, as the reference blends chroma before luma.
+#pragma once
+
+#include "common.hpp"
+
+namespace oxav1 {
+
+constexpr int kGrainW = 82, kGrainH = 73;
+
+struct FgScratch {
+    int16_t luma[kGrainH * kGrainW];
+    int16_t cb[kGrainH * kGrainW];
+    int16_t cr[kGrainH * kGrainW];
+    uint8_t lut[3][256];
+    int32_t chroma_w, chroma_h;
+    uint32_t nstripes, nblocks;
+    uint8_t *offsets;  // nstripes * nblocks LFSR draws
+};
+
+__constant__ int16_t c_gauss[2048];       // filled from kGaussianSequence at init
+__constant__ uint16_t c_lfsr_jump[16][16]; // [b][i]: column i of the step matrix ^ (2^b)
+
+// RandomRegister::next (film_grain.rs:117-130)
+__device__ __forceinline__ uint32_t fg_step(uint32_t &r) {
+    const uint32_t bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1u;
+    r = ((r >> 1) | (bit << 15)) & 0xffffu;
+    return r;
+}
+// state after n steps
+__device__ __forceinline__ uint32_t fg_jump(uint32_t r, uint32_t n) {
+    for (int b = 0; b < 16; b++) {
+        if (!((n >> b) & 1u)) continue;
+        uint32_t v = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if ((r >> i) & 1u) v ^= c_lfsr_jump[b][i];
+        r = v;
+    }
+    return r;
+}
+
+__device__ __forceinline__ int32_t fg_round2(int32_t x, uint32_t n) { return n == 0 ? x : (x + (1 << (n - 1))) >> n; }
+
+// scaling_lookup_fill (film_grain.rs:480-526), one lane per plane
+__device__ void fg_lut_fill(uint8_t *t, uint32_t np_, const uint8_t *xs, const uint8_t *ys) {
+    if (np_ == 0) {
+        for (int v = 0; v < 256; v++) t[v] = 0;
+        return;
+    }
+    const uint32_t x0 = xs[0];
+    for (uint32_t v = 0; v < min(x0, 256u); v++) t[v] = ys[0];
+    for (uint32_t i = 0; i + 1 < np_; i++) {
+        const int32_t xa = xs[i], xb = xs[i + 1], ya = ys[i], yb = ys[i + 1];
+        const int32_t dx = xb - xa;
+        if (dx <= 0) continue;
+        const int32_t delta = (yb - ya) * ((65536 + (dx >> 1)) / dx);
+        for (int32_t x = 0; x < dx; x++) {
+            const int32_t idx = xa + x;
+            if (idx < 256) t[idx] = (uint8_t)clip3(0, 255, ya + ((x * delta + 32768) >> 16));
+        }
+    }
+    const uint32_t xn = xs[np_ - 1];
+    for (uint32_t v = min(xn, 256u); v < 256; v++) t[v] = ys[np_ - 1];
+}
+
+// White noise for one template (generate_grain white-noise loops).
+__device__ void fg_white(int16_t *g, int count, uint32_t seed, bool on, uint32_t shift) {
+    const int nt = blockDim.x;
+    const int chunk = (count + nt - 1) / nt;
+    const int k0 = threadIdx.x * chunk;
+    if (!on) {
+        for (int k = k0; k < min(k0 + chunk, count); k++) g[k] = 0;
+        return;
+    }
+    uint32_t r = fg_jump(seed & 0xffffu, (uint32_t)k0);
+    for (int k = k0; k < min(k0 + chunk, count); k++) {
+        fg_step(r);
+        g[k] = (int16_t)fg_round2((int32_t)c_gauss[(r >> 5) & 2047u], shift);
+    }
+}
+
+// One workgroup per frame (blockDim.x >= 128).
+__device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) {
+    FgScratch *S = J.fgs;
+    const oxav1_film_grain_params &fg = J.fg;
+    const int tid = threadIdx.x;
+    const uint32_t bd = J.bd, sx = J.sx, sy = J.sy;
+    const int32_t gc = 128 << (bd - 8);
+    const int32_t gmin = -gc, gmax = (256 << (bd - 8)) - 1 - gc;
+    const uint32_t shift = (12u - bd) + fg.grain_scale_shift;
+    const int cw = sx ? 44 : 82, ch = sy ? 38 : 73;
+    int16_t *L = sm, *Cb = sm + kGrainH * kGrainW, *Cr = Cb + kGrainH * kGrainW;
+    const uint32_t np_ = J.np < 3 ? J.np : 3;
+    // scaling LUTs (scaling_lookup_init, film_grain.rs:418-466)
+    if (tid < 3) {
+        uint8_t *t = S->lut[tid];
+        if ((uint32_t)tid >= np_)
+            for (int v = 0; v < 256; v++) t[v] = 0;
+        else if (tid == 0 || fg.chroma_scaling_from_luma)
+            fg_lut_fill(t, fg.num_y_points, fg.point_y_value, fg.point_y_scaling);
+        else if (tid == 1)
+            fg_lut_fill(t, fg.num_cb_points, fg.point_cb_value, fg.point_cb_scaling);
+        else
+            fg_lut_fill(t, fg.num_cr_points, fg.point_cr_value, fg.point_cr_scaling);
+    }
+    // per-stripe block offsets (film_grain.rs:626-641)
+    for (uint32_t s = tid; s < S->nstripes; s += blockDim.x) {
+        uint32_t r = fg.grain_seed;
+        r ^= ((s * 37u + 178u) & 255u) << 8;
+        r ^= (s * 173u + 105u) & 255u;
+        for (uint32_t b = 0; b < S->nblocks; b++) {
+            fg_step(r);
+            S->offsets[s * S->nblocks + b] = (uint8_t)((r >> 8) & 255u);
+        }
+    }
+    if (tid == 0) {
+        S->chroma_w = cw;
+        S->chroma_h = ch;
+    }
+    // white noise
+    fg_white(L, kGrainH * kGrainW, fg.grain_seed, fg.num_y_points > 0, shift);
+    const bool chroma_on_cb = fg.num_cb_points > 0 || fg.chroma_scaling_from_luma;
+    const bool chroma_on_cr = fg.num_cr_points > 0 || fg.chroma_scaling_from_luma;
+    fg_white(Cb, cw * ch, fg.grain_seed ^ 0xb524u, chroma_on_cb, shift);
+    fg_white(Cr, cw * ch, fg.grain_seed ^ 0x49d8u, chroma_on_cr, shift);
+    __syncthreads();
+    const int32_t lag = fg.ar_coeff_lag;
+    const uint32_t ash = fg.ar_coeff_shift;
+    // luma AR wavefront: lane j owns row 3+j; at step t it filters column 3+t-4j
+    if (fg.num_y_points > 0) {
+        const int rows = kGrainH - 3, colsn = kGrainW - 6;
+        const int steps = colsn + 4 * (rows - 1);
+        for (int t = 0; t < steps; t++) {
+            const int j = tid;
+            if (j < rows) {
+                const int x = 3 + t - 4 * j;
+                if (x >= 3 && x < kGrainW - 3) {
+                    const int y = 3 + j;
+                    int32_t s = 0;
+                    int pos = 0;
+                    for (int dr = -lag; dr <= 0; dr++)
+                        for (int dc = -lag; dc <= lag; dc++) {
+                            if (dr == 0 && dc == 0) goto done_l;
+                            s += (int32_t)L[(y + dr) * kGrainW + (x + dc)] * ((int32_t)fg.ar_coeffs_y_plus_128[pos] - 128);
+                            pos++;
+                        }
+                done_l:
+                    L[y * kGrainW + x] = (int16_t)clip3(gmin, gmax, (int32_t)L[y * kGrainW + x] + fg_round2(s, ash));
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // chroma AR (Cb and Cr together), film_grain.rs:319-372
+    if (fg.chroma_scaling_from_luma || fg.num_cb_points > 0 || fg.num_cr_points > 0) {
+        const int rows = ch - 3, colsn = cw - 6;
+        const int steps = colsn + 4 * (rows - 1);
+        for (int t = 0; t < steps; t++) {
+            const int j = tid;
+            if (j < rows) {
+                const int x = 3 + t - 4 * j;
+                if (x >= 3 && x < cw - 3) {
+                    const int y = 3 + j;
+                    int32_t s0 = 0, s1 = 0;
+                    int pos = 0;
+                    for (int dr = -lag; dr <= 0; dr++)
+                        for (int dc = -lag; dc <= lag; dc++) {
+                            const int32_t c0 = (int32_t)fg.ar_coeffs_cb_plus_128[pos] - 128;
+                            const int32_t c1 = (int32_t)fg.ar_coeffs_cr_plus_128[pos] - 128;
+                            if (dr == 0 && dc == 0) {
+                                if (fg.num_y_points > 0) {
+                                    const int lx = ((x - 3) << sx) + 3, ly = ((y - 3) << sy) + 3;
+                                    int32_t l = 0;
+                                    for (int i = 0; i <= (int)sy; i++)
+                                        for (int k = 0; k <= (int)sx; k++) l += (int32_t)L[(ly + i) * kGrainW + (lx + k)];
+                                    l = fg_round2(l, sx + sy);
+                                    s0 += l * c0;
+                                    s1 += l * c1;
+                                }
+                                goto done_c;
+                            }
+                            s0 += (int32_t)Cb[(y + dr) * cw + (x + dc)] * c0;
+                            s1 += (int32_t)Cr[(y + dr) * cw + (x + dc)] * c1;
+                            pos++;
+                        }
+                done_c:
+                    Cb[y * cw + x] = (int16_t)clip3(gmin, gmax, (int32_t)Cb[y * cw + x] + fg_round2(s0, ash));
+                    Cr[y * cw + x] = (int16_t)clip3(gmin, gmax, (int32_t)Cr[y * cw + x] + fg_round2(s1, ash));
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < kGrainH * kGrainW; i += blockDim.x) S->luma[i] = L[i];
+    for (int i = tid; i < cw * ch; i += blockDim.x) {
+        S->cb[i] = Cb[i];
+        S->cr[i] = Cr[i];
+    }
+}
+
+// Grain sample of (stripe s, block b, row i, col j) for a plane.
+__device__ __forceinline__ int32_t fg_grain(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+                                            uint32_t s, uint32_t b, int32_t i, int32_t j, uint32_t psx, uint32_t psy) {
+    const uint32_t rnd = offs[s * nblocks + b];
+    const int32_t ox = (int32_t)(rnd >> 4), oy = (int32_t)(rnd & 15u);
+    const int32_t pox = psx ? 6 + ox : 9 + 2 * ox;
+    const int32_t poy = psy ? 6 + oy : 9 + 2 * oy;
+    return tmpl[(poy + i) * tw + pox + j];
+}
+
+// Value of noiseStripe[s][plane][i][x] after the horizontal overlap blend.
+__device__ __forceinline__ int32_t fg_hval(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+                                           uint32_t s, int32_t i, int32_t x, uint32_t psx, uint32_t psy, bool overlap,
+                                           int32_t gmin, int32_t gmax) {
+    if (!psx) {
+        const uint32_t b = (uint32_t)x >> 5;
+        const int32_t j = x & 31;
+        int32_t g = fg_grain(tmpl, tw, offs, nblocks, s, b, i, j, psx, psy);
+        if (j < 2 && b > 0 && overlap) {
+            const int32_t old = fg_grain(tmpl, tw, offs, nblocks, s, b - 1, i, j + 32, psx, psy);
+            g = j == 0 ? old * 27 + g * 17 : old * 17 + g * 27;
+            g = clip3(gmin, gmax, fg_round2(g, 5));
+        }
+        return g;
+    }
+    const uint32_t b = (uint32_t)x >> 4;
+    const int32_t j = x & 15;
+    int32_t g = fg_grain(tmpl, tw, offs, nblocks, s, b, i, j, psx, psy);
+    if (j == 0 && b > 0 && overlap) {
+        const int32_t old = fg_grain(tmpl, tw, offs, nblocks, s, b - 1, i, 16, psx, psy);
+        g = clip3(gmin, gmax, fg_round2(old * 23 + g * 22, 5));
+    }
+    return g;
+}
+
+// noiseImage[plane][y][x] (film_grain.rs:705-739)
+__device__ __forceinline__ int32_t fg_noise(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+                                            int32_t y, int32_t x, uint32_t psx, uint32_t psy, bool overlap,
+                                            int32_t gmin, int32_t gmax) {
+    const uint32_t s = (uint32_t)y >> (5 - psy);
+    const int32_t i = y - (int32_t)(s << (5 - psy));
+    int32_t g = fg_hval(tmpl, tw, offs, nblocks, s, i, x, psx, psy, overlap, gmin, gmax);
+    if (!psy) {
+        if (i < 2 && s > 0 && overlap) {
+            const int32_t old = fg_hval(tmpl, tw, offs, nblocks, s - 1, i + 32, x, psx, psy, overlap, gmin, gmax);
+            g = i == 0 ? old * 27 + g * 17 : old * 17 + g * 27;
+            g = clip3(gmin, gmax, fg_round2(g, 5));
+        }
+    } else if (i < 1 && s > 0 && overlap) {
+        const int32_t old = fg_hval(tmpl, tw, offs, nblocks, s - 1, i + 16, x, psx, psy, overlap, gmin, gmax);
+        g = clip3(gmin, gmax, fg_round2(old * 23 + g * 22, 5));
+    }
+    return g;
+}
+
+// scale_lut (film_grain.rs:543-555)
+__device__ __forceinline__ int32_t fg_scale(const uint8_t *t, int32_t index, uint32_t bd) {
+    const uint32_t shift = bd - 8;
+    const uint32_t x = (uint32_t)(index >> shift);
+    if (bd == 8 || x >= 255) return t[min(x, 255u)];
+    const int32_t rem = index - (int32_t)(x << shift);
+    const int32_t a = t[x], b = t[x + 1];
+    return a + fg_round2((b - a) * rem, shift);
+}
+
+// Blend rows [y0, y1) of one plane: in (pre-grain) -> out.
+template <typename T>
+__device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
+                              const uint8_t *lut /* LDS [3][256] */) {
+    const FgScratch *S = J.fgs;
+    const oxav1_film_grain_params &fg = J.fg;
+    const uint32_t bd = J.bd;
+    const uint32_t psx = plane ? J.sx : 0, psy = plane ? J.sy : 0;
+    const int32_t gc = 128 << (bd - 8);
+    const int32_t gmin = -gc, gmax = (256 << (bd - 8)) - 1 - gc;
+    int32_t minv, maxl, maxc;
+    if (fg.clip_to_restricted_range) {
+        minv = 16 << (bd - 8);
+        maxl = 235 << (bd - 8);
+        maxc = J.mc == 0 ? maxl : (240 << (bd - 8));
+    } else {
+        minv = 0;
+        maxl = maxc = (256 << (bd - 8)) - 1;
+    }
+    const uint32_t gs = fg.grain_scaling;
+    const bool overlap = fg.overlap_flag != 0;
+    const T *in = plane_ptr<T>(J.ref_out[plane]);
+    T *out = plane_ptr<T>(J.out[plane]);
+    const uint32_t ip = J.ref_pitch[plane], op = J.out_pitch[plane];
+    const int32_t w = (int32_t)J.out_cols[0];
+    const int32_t cols = x1 - x0;
+    bool active;
+    const int16_t *tmpl;
+    int tw;
+    if (plane == 0) {
+        active = fg.num_y_points > 0;
+        tmpl = S->luma;
+        tw = kGrainW;
+    } else {
+        const bool any = fg.num_cb_points > 0 || fg.num_cr_points > 0 || fg.chroma_scaling_from_luma;
+        active = J.np >= 3 && any &&
+                 (plane == 1 ? (fg.num_cb_points > 0 || fg.chroma_scaling_from_luma)
+                             : (fg.num_cr_points > 0 || fg.chroma_scaling_from_luma));
+        tmpl = plane == 1 ? S->cb : S->cr;
+        tw = S->chroma_w;
+    }
+    const T *ly = plane_ptr<T>(J.ref_out[0]);
+    const uint32_t lp = J.ref_pitch[0];
+    for (int32_t i = threadIdx.x; i < (y1 - y0) * cols; i += blockDim.x) {
+        const int32_t y = y0 + i / cols, x = x0 + i % cols;
+        const int32_t orig = (int32_t)in[(size_t)y * ip + x];
+        if (!active) {
+            out[(size_t)y * op + x] = (T)orig;
+            continue;
+        }
+        const int32_t nz = fg_noise(tmpl, tw, S->offsets, S->nblocks, y, x, psx, psy, overlap, gmin, gmax);
+        if (plane == 0) {
+            const int32_t n2 = fg_round2(fg_scale(lut, orig, bd) * nz, gs);
+            out[(size_t)y * op + x] = (T)clip3(minv, maxl, orig + n2);
+            continue;
+        }
+        const int32_t lx = x << psx, lyy = y << psy;
+        const int32_t lnx = min(lx + 1, w - 1);
+        const int32_t avg = psx ? fg_round2((int32_t)ly[(size_t)lyy * lp + lx] + (int32_t)ly[(size_t)lyy * lp + lnx], 1)
+                                : (int32_t)ly[(size_t)lyy * lp + lx];
+        int32_t merged;
+        if (fg.chroma_scaling_from_luma) {
+            merged = avg;
+        } else {
+            const int32_t lm = plane == 1 ? fg.cb_luma_mult : fg.cr_luma_mult;
+            const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
+            const int32_t off = plane == 1 ? fg.cb_offset : fg.cr_offset;
+            const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
+            merged = clip3(0, (1 << bd) - 1, (comb >> 6) + ((off - 256) << (bd - 8)));
+        }
+        const int32_t n2 = fg_round2(fg_scale(lut + 256 * plane, merged, bd) * nz, gs);
+        out[(size_t)y * op + x] = (T)clip3(minv, maxc, orig + n2);
+    }
+}
+
+}  // namespace oxav1

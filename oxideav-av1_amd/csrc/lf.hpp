@@ -1,0 +1,304 @@
+// §7.14 deblocking on gfx950: one workgroup per 64x64 plane tile, both
+// passes fused, out of place (src -> deb).
+//
+// Reference: loop_filter.rs:271-904.  The reference walks every edge of a
+// plane sequentially and in place: all vertical edges (pass 0), then all
+// horizontal edges (pass 1).  For legal block/transform geometry the used
+// read/write footprints of different edges of one pass never overlap
+// (an edge of filter size s reads +-s/2 and writes +-(s/2-1) inside the two
+// transforms it separates), so every edge of a pass can be filtered at
+// once.  A tile therefore loads its 64x64 region plus a 12-sample halo into
+// LDS, runs pass 0 on every loaded row for the vertical edges whose writes
+// reach the tile's columns, runs pass 1 for the horizontal edges whose writes
+// reach the tile's rows, and stores only the tile itself.  Halo samples are
+// recomputed by neighbouring tiles from the same source, so tiles are
+// independent and no second launch or grid barrier is needed.
+//
+// PlaneBuffer semantics are kept: reads clamp to the plane, writes outside
+// the plane are dropped (loop_filter.rs:128-158).
+#pragma once
+
+#include "av1_tables.h"
+#include "common.hpp"
+
+namespace oxav1 {
+
+constexpr int kLfTile = 64;
+constexpr int kLfHalo = 12;
+constexpr int kLfRegion = kLfTile + 2 * kLfHalo;   // 88
+constexpr int kLfEdges = kLfTile / 4 + 3;          // edges at X0-4 .. X0+TW+4
+constexpr int kLfCellRows = kLfRegion / 4;         // 22
+
+__constant__ uint8_t c_tx_w[19] = {4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 4, 16, 8, 32, 16, 64};
+__constant__ uint8_t c_tx_h[19] = {4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 4, 32, 8, 64, 16};
+__constant__ uint8_t c_num4x4_w[22] = {1, 1, 2, 2, 2, 4, 4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 1, 4, 2, 8, 4, 16};
+__constant__ uint8_t c_num4x4_h[22] = {1, 2, 1, 2, 4, 2, 4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 4, 1, 8, 2, 16, 4};
+__constant__ uint8_t c_subsampled[88];  // filled from kSubsampledSize at init
+
+// adaptive_filter_strength_selection (loop_filter.rs:527-572) for the cell's
+// packed word; returns lvl.
+__device__ __forceinline__ uint32_t lf_level_at(const FrameJob &J, uint32_t row, uint32_t col,
+                                                uint32_t plane, uint32_t pass) {
+    const size_t k = (size_t)row * J.mi_cols + col;
+    const uint32_t w = J.lf_mi[k];
+    const uint32_t seg = w >> OXAV1_MI_SEG_SHIFT;
+    const uint32_t ref = (w >> OXAV1_MI_REF_SHIFT) & 15u;
+    const uint32_t mode = (w >> OXAV1_MI_MODE_SHIFT) & 31u;
+    const uint32_t mode_type = (mode >= 14 && mode != 16 && mode != 24) ? 1u : 0u;
+    const uint32_t i = plane == 0 ? pass : plane + 1;
+    const uint32_t dli = J.lf_delta_multi ? i : 0u;
+    const int32_t dlf = J.delta_lf ? (int32_t)J.delta_lf[k * 4 + dli] : 0;
+    int32_t lvl = clip3(0, 63, dlf + (int32_t)J.lf_level[i]);
+    const uint32_t feature = 1 + i;
+    if (seg < 8 && J.seg_active[seg][feature]) lvl = clip3(0, 63, (int32_t)J.seg_data[seg][feature] + lvl);
+    if (J.lf_delta_enabled) {
+        const int32_t n_shift = lvl >> 5;
+        if (ref == 0) {
+            lvl += (int32_t)J.lf_ref_deltas[0] << n_shift;
+        } else {
+            const uint32_t ri = ref < 7 ? ref : 7;
+            lvl += (int32_t)J.lf_ref_deltas[ri] << n_shift;
+            lvl += (int32_t)J.lf_mode_deltas[mode_type] << n_shift;
+        }
+        lvl = clip3(0, 63, lvl);
+    }
+    return (uint32_t)lvl;
+}
+
+// loop_filter_edge (loop_filter.rs:302-397) up to the strength: returns 0 for
+// "no filtering", else (filter_size | lvl << 8).  (by, bx) index the plane's
+// 4x4 blocks, i.e. the loop cursor row = by << sy, col = bx << sx
+// (loop_filter.rs:282-292).
+__device__ uint32_t lf_decide(const FrameJob &J, uint32_t plane, uint32_t pass, int32_t by,
+                              int32_t bx) {
+    if (by < 0 || bx < 0) return 0;
+    const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
+    uint32_t row = (uint32_t)by << sy, col = (uint32_t)bx << sx;
+    if (row >= J.mi_rows || col >= J.mi_cols) return 0;  // not visited by the walk
+    const uint32_t x = col * 4, y = row * 4;
+    row |= sy;
+    col |= sx;
+    if (x >= J.fw || y >= J.fh) return 0;
+    if (pass == 0 && x == 0) return 0;
+    if (pass == 1 && y == 0) return 0;
+    const uint32_t xp = x >> sx, yp = y >> sy;
+    const int32_t prev_row = (int32_t)row - ((pass == 1 ? 1 : 0) << sy);
+    const int32_t prev_col = (int32_t)col - ((pass == 0 ? 1 : 0) << sx);
+    if (prev_row < 0 || prev_col < 0) return 0;
+    if (row >= J.mi_rows || col >= J.mi_cols) return 0;
+    const uint32_t w = J.lf_mi[(size_t)row * J.mi_cols + col];
+    const uint32_t mi_size = (w >> OXAV1_MI_SIZE_SHIFT) & 31u;
+    const uint32_t tcols = J.tx_cols[plane];
+    const uint32_t tx_sz = J.tx[plane][(size_t)(row >> sy) * tcols + (col >> sx)];
+    const uint32_t ptx = J.tx[plane][(size_t)((uint32_t)prev_row >> sy) * tcols + ((uint32_t)prev_col >> sx)];
+    // planeSize = get_plane_residual_size(MiSize, plane).unwrap_or(MiSize)
+    uint32_t plane_size = mi_size;
+    if (mi_size < 22) {
+        const uint32_t subx = plane ? (J.sx != 0) : 0, suby = plane ? (J.sy != 0) : 0;
+        const uint32_t raw = c_subsampled[mi_size * 4 + subx * 2 + suby];
+        if (raw != OXAV1_BLOCK_INVALID) plane_size = raw;
+    }
+    const bool skip = (w >> OXAV1_MI_SKIP_BIT) & 1u;
+    const bool is_intra = (w >> OXAV1_MI_INTRA_BIT) & 1u;
+    bool block_edge, tx_edge;
+    uint32_t fs;
+    if (pass == 0) {
+        const uint32_t bw = plane_size < 22 ? (uint32_t)c_num4x4_w[plane_size] * 4 : 4u;
+        block_edge = (xp % bw) == 0;
+        tx_edge = (xp % c_tx_w[tx_sz]) == 0;
+        fs = min((uint32_t)c_tx_w[tx_sz], (uint32_t)c_tx_w[ptx]);
+    } else {
+        const uint32_t bh = plane_size < 22 ? (uint32_t)c_num4x4_h[plane_size] * 4 : 4u;
+        block_edge = (yp % bh) == 0;
+        tx_edge = (yp % c_tx_h[tx_sz]) == 0;
+        fs = min((uint32_t)c_tx_h[tx_sz], (uint32_t)c_tx_h[ptx]);
+    }
+    fs = min(fs, plane == 0 ? 16u : 8u);
+    const bool apply = tx_edge && (block_edge || !skip || is_intra);
+    if (!apply) return 0;
+    uint32_t lvl = lf_level_at(J, row, col, plane, pass);
+    if (lvl == 0) lvl = lf_level_at(J, (uint32_t)prev_row, (uint32_t)prev_col, plane, pass);
+    if (lvl == 0) return 0;
+    return fs | (lvl << 8);
+}
+
+// LDS view of the tile region with the reference's clamped reads / dropped
+// writes.  Region origin (ry0, rx0) may be negative; only in-plane entries are
+// loaded and every clamped coordinate lands inside the region.
+template <typename L>
+struct LfRegion {
+    L *s;
+    int32_t ry0, rx0, rows, cols;
+    __device__ __forceinline__ int idx(int32_t y, int32_t x) const {
+        y = clip3(0, rows - 1, y);
+        x = clip3(0, cols - 1, x);
+        return (y - ry0) * kLfRegion + (x - rx0);
+    }
+    __device__ __forceinline__ int32_t get(int32_t y, int32_t x) const { return (int32_t)s[idx(y, x)]; }
+    __device__ __forceinline__ void set(int32_t y, int32_t x, int32_t v) const {
+        if (y < 0 || x < 0 || y >= rows || x >= cols) return;
+        s[(y - ry0) * kLfRegion + (x - rx0)] = (L)v;
+    }
+};
+
+__device__ __forceinline__ int32_t f4clamp(int32_t v, int32_t lo, int32_t hi) { return clip3(lo, hi, v); }
+
+// sample_filtering + filter_mask + narrow/wide filter (loop_filter.rs:582-822)
+// for one sample position of one edge.
+template <typename L>
+__device__ void lf_filter_sample(const LfRegion<L> &R, int32_t x, int32_t y, int32_t dx, int32_t dy,
+                                 uint32_t plane, uint32_t fs, uint32_t lvl, uint32_t sharp, uint32_t bd) {
+    // strength (loop_filter.rs:502-523)
+    const int32_t shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
+    const int32_t limit = sharp > 0 ? clip3(1, 9 - (int32_t)sharp, (int32_t)lvl >> shift)
+                                    : max((int32_t)lvl >> shift, 1);
+    const int32_t blimit = 2 * ((int32_t)lvl + 2) + limit;
+    const int32_t thresh = (int32_t)lvl >> 4;
+    int32_t q[7], p[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        q[k] = R.get(y + k * dy, x + k * dx);
+        p[k] = R.get(y - (k + 1) * dy, x - (k + 1) * dx);
+    }
+    const uint32_t bds = bd - 8;
+    const int32_t thresh_bd = thresh << bds, limit_bd = limit << bds, blimit_bd = blimit << bds;
+    const bool hev = abs(p[1] - p[0]) > thresh_bd || abs(q[1] - q[0]) > thresh_bd;
+    const uint32_t flen = fs == 4 ? 4 : (plane != 0 ? 6 : (fs == 8 ? 8 : 16));
+    bool m = abs(p[1] - p[0]) > limit_bd;
+    m |= abs(q[1] - q[0]) > limit_bd;
+    m |= abs(p[0] - q[0]) * 2 + abs(p[1] - q[1]) / 2 > blimit_bd;
+    if (flen >= 6) {
+        m |= abs(p[2] - p[1]) > limit_bd;
+        m |= abs(q[2] - q[1]) > limit_bd;
+    }
+    if (flen >= 8) {
+        m |= abs(p[3] - p[2]) > limit_bd;
+        m |= abs(q[3] - q[2]) > limit_bd;
+    }
+    if (m) return;  // filterMask == 0
+    const int32_t one = 1 << bds;
+    bool flat = false, flat2 = false;
+    if (fs >= 8) {
+        bool f = abs(p[1] - p[0]) > one || abs(q[1] - q[0]) > one || abs(p[2] - p[0]) > one ||
+                 abs(q[2] - q[0]) > one;
+        if (flen >= 8) f = f || abs(p[3] - p[0]) > one || abs(q[3] - q[0]) > one;
+        flat = !f;
+    }
+    if (fs >= 16) {
+        bool f = abs(p[6] - p[0]) > one || abs(q[6] - q[0]) > one || abs(p[5] - p[0]) > one ||
+                 abs(q[5] - q[0]) > one || abs(p[4] - p[0]) > one || abs(q[4] - q[0]) > one;
+        flat2 = !f;
+    }
+    if (fs == 4 || !flat) {
+        // narrow_filter (loop_filter.rs:743-782)
+        const int32_t lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1;
+        const int32_t half = 0x80 << bds;
+        const int32_t ps1 = p[1] - half, ps0 = p[0] - half, qs0 = q[0] - half, qs1 = q[1] - half;
+        int32_t filt = hev ? f4clamp(ps1 - qs1, lo, hi) : 0;
+        filt = f4clamp(filt + 3 * (qs0 - ps0), lo, hi);
+        const int32_t f1 = f4clamp(filt + 4, lo, hi) >> 3;
+        const int32_t f2 = f4clamp(filt + 3, lo, hi) >> 3;
+        R.set(y, x, f4clamp(qs0 - f1, lo, hi) + half);
+        R.set(y - dy, x - dx, f4clamp(ps0 + f2, lo, hi) + half);
+        if (!hev) {
+            const int32_t f = (f1 + 1) >> 1;
+            R.set(y + dy, x + dx, f4clamp(qs1 - f, lo, hi) + half);
+            R.set(y - 2 * dy, x - 2 * dx, f4clamp(ps1 + f, lo, hi) + half);
+        }
+        return;
+    }
+    // wide_filter (loop_filter.rs:789-822); tap index p = Clip3(-(n+1), n, i+j)
+    const uint32_t log2 = (fs == 8 || !flat2) ? 3 : 4;
+    const int32_t n = log2 == 4 ? 6 : (plane == 0 ? 3 : 2);
+    const int32_t n2 = (log2 == 3 && plane == 0) ? 0 : 1;
+    // v[t] for t = -7..6 -> index t+7: p[6..0] then q[0..6]
+    int32_t v[14];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        v[6 - k] = p[k];
+        v[7 + k] = q[k];
+    }
+    int32_t out[12];
+#pragma unroll
+    for (int ii = 0; ii < 12; ii++) {
+        const int32_t i = ii - n;
+        if (i >= n) break;
+        int32_t t = 0;
+        for (int32_t j = -n; j <= n; j++) {
+            const int32_t pidx = clip3(-(n + 1), n, i + j);
+            const int32_t tap = abs(j) <= n2 ? 2 : 1;
+            t += v[pidx + 7] * tap;
+        }
+        out[ii] = (t + (1 << (log2 - 1))) >> log2;
+    }
+#pragma unroll
+    for (int ii = 0; ii < 12; ii++) {
+        const int32_t i = ii - n;
+        if (i >= n) break;
+        R.set(y + i * dy, x + i * dx, out[ii]);
+    }
+}
+
+// One 64x64 tile of one plane: src (T) -> dst (T).
+template <typename T>
+__device__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t tx,
+                        const T *__restrict__ src, uint32_t spitch, T *__restrict__ dst,
+                        uint32_t dpitch, typename LdsOf<T>::type *sm, uint16_t *dec0, uint16_t *dec1) {
+    using L = typename LdsOf<T>::type;
+    const int32_t rows = (int32_t)J.pad_rows[plane], cols = (int32_t)J.pad_cols[plane];
+    const int32_t Y0 = (int32_t)ty * kLfTile, X0 = (int32_t)tx * kLfTile;
+    LfRegion<L> R{sm, Y0 - kLfHalo, X0 - kLfHalo, rows, cols};
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    // 1. load the in-plane part of the region
+    for (int i = tid; i < kLfRegion * kLfRegion; i += nt) {
+        const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
+        const int32_t y = R.ry0 + ry, x = R.rx0 + rx;
+        if (y >= 0 && y < rows && x >= 0 && x < cols) sm[i] = (L)src[(size_t)y * spitch + x];
+    }
+    // 2. edge decisions: pass 0 for region cell rows x edges X0-4..X0+TW+4,
+    //    pass 1 for edge rows Y0-4..Y0+TH+4 x tile cell columns.
+    const uint32_t pass_dis = (J.apply_lf && J.lf_plane_on[plane]) ? 0u : 1u;
+    for (int i = tid; i < kLfCellRows * kLfEdges; i += nt) {
+        const int cr = i / kLfEdges, e = i - cr * kLfEdges;
+        const int32_t by = (R.ry0 >> 2) + cr, bx = ((X0 - 4) >> 2) + e;
+        dec0[i] = pass_dis ? 0 : (uint16_t)lf_decide(J, plane, 0, by, bx);
+    }
+    for (int i = tid; i < kLfEdges * (kLfTile / 4); i += nt) {
+        const int er = i / (kLfTile / 4), c = i - er * (kLfTile / 4);
+        const int32_t by = ((Y0 - 4) >> 2) + er, bx = (X0 >> 2) + c;
+        dec1[i] = pass_dis ? 0 : (uint16_t)lf_decide(J, plane, 1, by, bx);
+    }
+    __syncthreads();
+    const uint32_t sharp = J.lf_sharp, bd = J.bd;
+    // 3. pass 0: every loaded row x vertical edge
+    for (int i = tid; i < kLfRegion * kLfEdges; i += nt) {
+        const int ry = i / kLfEdges, e = i - ry * kLfEdges;
+        const int32_t y = R.ry0 + ry;
+        if (y < 0 || y >= rows) continue;
+        const uint32_t d = dec0[(ry >> 2) * kLfEdges + e];
+        if (!d) continue;
+        const int32_t x = X0 - 4 + 4 * e;
+        lf_filter_sample(R, x, y, 1, 0, plane, d & 0xffu, d >> 8, sharp, bd);
+    }
+    __syncthreads();
+    // 4. pass 1: horizontal edges x tile columns
+    for (int i = tid; i < kLfEdges * kLfTile; i += nt) {
+        const int er = i / kLfTile, c = i - er * kLfTile;
+        const int32_t x = X0 + c;
+        if (x >= cols) continue;
+        const uint32_t d = dec1[er * (kLfTile / 4) + (c >> 2)];
+        if (!d) continue;
+        const int32_t y = Y0 - 4 + 4 * er;
+        lf_filter_sample(R, x, y, 0, 1, plane, d & 0xffu, d >> 8, sharp, bd);
+    }
+    __syncthreads();
+    // 5. store the tile
+    for (int i = tid; i < kLfTile * kLfTile; i += nt) {
+        const int yy = i / kLfTile, xx = i - yy * kLfTile;
+        const int32_t y = Y0 + yy, x = X0 + xx;
+        if (y < rows && x < cols) dst[(size_t)y * dpitch + x] = (T)sm[(y - R.ry0) * kLfRegion + (x - R.rx0)];
+    }
+}
+
+}  // namespace oxav1

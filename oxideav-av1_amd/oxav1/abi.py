@@ -1,0 +1,236 @@
+"""ctypes view of the C ABI in include/oxav1_pf.h and include/oxav1_gen.h.
+
+The struct layouts mirror the C declarations field for field; every struct
+is checked against the library's own sizeof (oxav1_pf_struct_size) when the
+library is loaded.  The shared library is built in-tree
+(oxideav-av1_amd/liboxav1_pf.so); loading fails loudly if it is missing.
+"""
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "liboxav1_pf.so")
+
+OK = 0
+ERR_UPSCALED_NOT_LARGER = 1
+ERR_PLANE_SHAPE_MISMATCH = 2
+ERR_PLANE_COUNT_MISMATCH = 3
+ERR_INVALID_ARG = 10
+ERR_ALIAS = 11
+ERR_HIP = 12
+ERR_OUT_OF_MEMORY = 13
+ERR_ADMISSION_REFUSED = 14
+ERR_UNSUPPORTED = 15
+ERR_NO_DEVICE = 16
+
+RESTORE_NONE, RESTORE_WIENER, RESTORE_SGRPROJ, RESTORE_SWITCHABLE = 0, 1, 2, 3
+MI_SIZE_SHIFT, MI_SKIP_BIT, MI_INTRA_BIT, MI_REF_SHIFT, MI_MODE_SHIFT, MI_SEG_SHIFT = 0, 5, 6, 8, 12, 24
+
+u8, i8, u16, i16, u32, i32, u64 = (C.c_uint8, C.c_int8, C.c_uint16, C.c_int16, C.c_uint32,
+                                   C.c_int32, C.c_uint64)
+
+
+class Plane(C.Structure):
+    _fields_ = [("samples", C.POINTER(i32)), ("rows", u32), ("cols", u32)]
+
+
+class LfCtx(C.Structure):
+    _fields_ = [
+        ("loop_filter_level", u8 * 4), ("loop_filter_sharpness", u8),
+        ("loop_filter_delta_enabled", u8), ("delta_lf_multi", u8), ("num_planes", u8),
+        ("loop_filter_ref_deltas", i8 * 8), ("loop_filter_mode_deltas", i8 * 2),
+        ("bit_depth", u8), ("subsampling_x", u8), ("subsampling_y", u8), ("reserved0", u8),
+        ("mi_rows", u32), ("mi_cols", u32), ("frame_width", u32), ("frame_height", u32),
+    ]
+
+
+class LfGrids(C.Structure):
+    _fields_ = [
+        ("is_intra", C.POINTER(u8)), ("skip", C.POINTER(u8)), ("ref_frame", C.POINTER(u8)),
+        ("mode", C.POINTER(u8)), ("segment_id", C.POINTER(u8)), ("delta_lf", C.POINTER(i8)),
+        ("mi_size", C.POINTER(u8)), ("lf_tx_size", C.POINTER(u8) * 3),
+        ("seg_feature_active", (u8 * 8) * 8), ("seg_feature_data", (i16 * 8) * 8),
+    ]
+
+
+class CdefParams(C.Structure):
+    _fields_ = [
+        ("cdef_damping", u8), ("cdef_bits", u8), ("y_pri_strength", u8 * 8),
+        ("y_sec_strength", u8 * 8), ("uv_pri_strength", u8 * 8), ("uv_sec_strength", u8 * 8),
+    ]
+
+
+class CdefCtx(C.Structure):
+    _fields_ = [
+        ("mi_rows", u32), ("mi_cols", u32), ("num_planes", u8), ("bit_depth", u8),
+        ("subsampling_x", u8), ("subsampling_y", u8), ("params", CdefParams),
+    ]
+
+
+class CdefGrids(C.Structure):
+    _fields_ = [("cdef_idx", C.POINTER(i8)), ("skip", C.POINTER(u8))]
+
+
+class SrCtx(C.Structure):
+    _fields_ = [
+        ("use_superres", u8), ("num_planes", u8), ("bit_depth", u8), ("subsampling_x", u8),
+        ("subsampling_y", u8), ("reserved", u8 * 3), ("frame_width", u32),
+        ("upscaled_width", u32), ("frame_height", u32), ("mi_cols", u32),
+    ]
+
+
+class LrUnit(C.Structure):
+    _fields_ = [("type", u8), ("sgr_set", u8), ("wiener", (i16 * 3) * 2), ("sgr_xqd", i16 * 2)]
+
+
+class LrCtx(C.Structure):
+    _fields_ = [
+        ("mi_rows", u32), ("mi_cols", u32), ("num_planes", u8), ("bit_depth", u8),
+        ("subsampling_x", u8), ("subsampling_y", u8), ("frame_height", u32),
+        ("upscaled_width", u32), ("frame_restoration_type", u8 * 3), ("uses_lr", u8),
+        ("loop_restoration_size", u32 * 3),
+    ]
+
+
+class LrGrids(C.Structure):
+    _fields_ = [("units", C.POINTER(LrUnit) * 3)]
+
+
+class FilmGrainParams(C.Structure):
+    _fields_ = [
+        ("apply_grain", u8), ("update_grain", u8), ("grain_seed", u16), ("num_y_points", u8),
+        ("point_y_value", u8 * 14), ("point_y_scaling", u8 * 14),
+        ("chroma_scaling_from_luma", u8), ("num_cb_points", u8), ("point_cb_value", u8 * 10),
+        ("point_cb_scaling", u8 * 10), ("num_cr_points", u8), ("point_cr_value", u8 * 10),
+        ("point_cr_scaling", u8 * 10), ("grain_scaling", u8), ("ar_coeff_lag", u8),
+        ("ar_coeffs_y_plus_128", u8 * 24), ("ar_coeffs_cb_plus_128", u8 * 25),
+        ("ar_coeffs_cr_plus_128", u8 * 25), ("ar_coeff_shift", u8), ("grain_scale_shift", u8),
+        ("cb_mult", u8), ("cb_luma_mult", u8), ("cb_offset", u16), ("cr_mult", u8),
+        ("cr_luma_mult", u8), ("cr_offset", u16), ("overlap_flag", u8),
+        ("clip_to_restricted_range", u8),
+    ]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [
+        ("frame_width", u32), ("frame_height", u32), ("upscaled_width", u32), ("mi_rows", u32),
+        ("mi_cols", u32), ("bit_depth", u8), ("subsampling_x", u8), ("subsampling_y", u8),
+        ("num_planes", u8), ("matrix_coefficients", u8), ("use_superres", u8), ("apply_lf", u8),
+        ("apply_cdef", u8), ("lf", LfCtx), ("cdef", CdefParams), ("lr", LrCtx),
+        ("fg", FilmGrainParams), ("seg_feature_active", (u8 * 8) * 8),
+        ("seg_feature_data", (i16 * 8) * 8),
+    ]
+
+
+class FrameDev(C.Structure):
+    _fields_ = [
+        ("src", C.c_void_p * 3), ("src_pitch", u32 * 3), ("ref_out", C.c_void_p * 3),
+        ("ref_pitch", u32 * 3), ("out", C.c_void_p * 3), ("out_pitch", u32 * 3),
+        ("lf_mi", C.c_void_p), ("delta_lf", C.c_void_p), ("lf_tx_size", C.c_void_p * 3),
+        ("cdef_idx", C.c_void_p), ("lr_units", C.c_void_p * 3),
+    ]
+
+
+class GenConfig(C.Structure):
+    _fields_ = [
+        ("upscaled_width", u32), ("frame_height", u32), ("bit_depth", u8), ("subsampling_x", u8),
+        ("subsampling_y", u8), ("mono", u8), ("sb128", u8), ("superres_denom", u8),
+        ("lr_unit_shift", u8), ("lr_uv_shift", u8), ("enable_lf", u8), ("enable_cdef", u8),
+        ("enable_lr", u8), ("enable_fg", u8), ("fg_overlap", u8), ("fg_num_y_points", u8),
+        ("fg_ar_lag", u8), ("fg_chroma_points", u8), ("noise_amp", u8), ("rect_partitions", u8),
+        ("exercise_segments", u8), ("exercise_delta_lf", u8), ("cdef_skip_units", u8),
+        ("lr_types", u8),
+    ]
+
+
+class GenFrame(C.Structure):
+    _fields_ = [
+        ("desc", FrameDesc), ("planes", C.POINTER(i32) * 3), ("plane_rows", u32 * 3),
+        ("plane_cols", u32 * 3), ("is_intra", C.POINTER(u8)), ("skip", C.POINTER(u8)),
+        ("ref_frame", C.POINTER(u8)), ("mode", C.POINTER(u8)), ("segment_id", C.POINTER(u8)),
+        ("mi_size", C.POINTER(u8)), ("delta_lf", C.POINTER(i8)), ("lf_tx_size", C.POINTER(u8) * 3),
+        ("tx_rows", u32 * 3), ("tx_cols", u32 * 3), ("lf_mi", C.POINTER(u32)),
+        ("cdef_idx", C.POINTER(i8)), ("cdef_rows", u32), ("cdef_cols", u32),
+        ("lr_units", C.POINTER(LrUnit) * 3), ("lr_unit_rows", u32 * 3), ("lr_unit_cols", u32 * 3),
+    ]
+
+
+ABI_STRUCTS = {
+    "oxav1_plane_i32": Plane, "oxav1_lf_ctx": LfCtx, "oxav1_lf_grids": LfGrids,
+    "oxav1_cdef_params": CdefParams, "oxav1_cdef_ctx": CdefCtx, "oxav1_cdef_grids": CdefGrids,
+    "oxav1_sr_ctx": SrCtx, "oxav1_lr_unit": LrUnit, "oxav1_lr_ctx": LrCtx,
+    "oxav1_lr_grids": LrGrids, "oxav1_film_grain_params": FilmGrainParams,
+    "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev,
+}
+
+_P = C.POINTER
+_SIGS = {
+    "oxav1_pf_init": (C.c_int, []),
+    "oxav1_pf_strerror": (C.c_char_p, [C.c_int]),
+    "oxav1_pf_last_hip_error": (C.c_char_p, []),
+    "oxav1_pf_table_checksum": (C.c_int64, [C.c_char_p]),
+    "oxav1_pf_struct_size": (C.c_size_t, [C.c_char_p]),
+    "oxav1_pf_count_units_in_frame": (u32, [u32, u32]),
+    "oxav1_pf_loop_filter_frame": (C.c_int, [_P(LfCtx), _P(LfGrids), _P(Plane), u32]),
+    "oxav1_pf_cdef_frame": (C.c_int, [_P(CdefCtx), _P(CdefGrids), _P(Plane), u32, _P(Plane), u32]),
+    "oxav1_pf_upscale_frame": (C.c_int, [_P(SrCtx), _P(Plane), u32, _P(Plane), u32]),
+    "oxav1_pf_loop_restoration_frame": (
+        C.c_int, [_P(LrCtx), _P(LrGrids), _P(Plane), u32, _P(Plane), u32, _P(Plane), u32]),
+    "oxav1_pf_film_grain_synthesis": (
+        C.c_int, [_P(FilmGrainParams), u8, u8, u8, u8, u8, _P(Plane), u32]),
+    "oxav1_pf_engine_create": (C.c_int, [_P(C.c_void_p), C.c_void_p, u32]),
+    "oxav1_pf_engine_destroy": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_engine_prepare": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev), u32]),
+    "oxav1_pf_engine_run": (C.c_int, [C.c_void_p, C.c_int]),
+    "oxav1_pf_engine_run_frame": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev)]),
+    "oxav1_pf_engine_run_batch": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev), u32, C.c_int]),
+    "oxav1_pf_engine_sync": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "oxav1_pf_engine_stage_ms": (C.c_int, [C.c_void_p, C.c_char_p, _P(C.c_double), _P(u32)]),
+    "oxav1_pf_engine_stream": (C.c_void_p, [C.c_void_p]),
+    "oxav1_gen_default_config": (C.c_int, [C.c_int, u64, _P(GenConfig)]),
+    "oxav1_gen_frame_create": (C.c_int, [_P(GenConfig), u64, _P(_P(GenFrame))]),
+    "oxav1_gen_frame_destroy": (None, [_P(GenFrame)]),
+    "oxav1_gen_splitmix64": (u64, [u64]),
+}
+
+_lib = None
+
+
+class Oxav1Error(RuntimeError):
+    def __init__(self, status, what=""):
+        lib = load()
+        msg = lib.oxav1_pf_strerror(status).decode()
+        if status == ERR_HIP:
+            msg += " (" + lib.oxav1_pf_last_hip_error().decode() + ")"
+        super().__init__(f"{what}: status {status}: {msg}" if what else f"status {status}: {msg}")
+        self.status = status
+
+
+def load(path=None):
+    """Load liboxav1_pf.so (built in-tree); raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `make -C oxideav-av1_amd` or __graft_entry__.build()")
+    lib = C.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for cname, cls in ABI_STRUCTS.items():
+        n = lib.oxav1_pf_struct_size(cname.encode())
+        if n != C.sizeof(cls):
+            raise RuntimeError(f"ABI mismatch for {cname}: C {n} bytes, ctypes {C.sizeof(cls)}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != OK:
+        raise Oxav1Error(status, what)
+    return status

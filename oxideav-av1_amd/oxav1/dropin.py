@@ -1,0 +1,93 @@
+"""Python-level calls of the five drop-in entry points.
+
+`FilterCalls` wraps any library exporting the five functions with the
+include/oxav1_pf.h signatures (the HIP library here; the CPU oracle binds
+the same class in oracle/oracle.py), taking and returning numpy int32
+planes so results of different implementations compare directly.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .frames import planes_struct
+
+GPU_NAMES = {
+    "lf": "oxav1_pf_loop_filter_frame",
+    "cdef": "oxav1_pf_cdef_frame",
+    "sr": "oxav1_pf_upscale_frame",
+    "lr": "oxav1_pf_loop_restoration_frame",
+    "fg": "oxav1_pf_film_grain_synthesis",
+}
+
+
+class FilterCalls:
+    def __init__(self, lib, names, status_returning=True):
+        self.lib = lib
+        self.fn = {k: getattr(lib, v) for k, v in names.items()}
+        self.status_returning = status_returning
+
+    def _st(self, ret, what):
+        if self.status_returning:
+            abi.check(ret, what)
+        return ret
+
+    # loop_filter_frame (loop_filter.rs:271), in place -> returns new arrays
+    def loop_filter_frame(self, frame, planes, ctx=None):
+        out = [np.ascontiguousarray(p, dtype=np.int32).copy() for p in planes]
+        c, g, keep = frame.lf_args()
+        if ctx is not None:
+            c = ctx
+        ps = planes_struct(out)
+        self._st(self.fn["lf"](C.byref(c), C.byref(g), ps, len(out)), "loop_filter_frame")
+        return out
+
+    # cdef_frame (cdef.rs:180): dst starts as `dst_init` (default: copy of src)
+    def cdef_frame(self, frame, src, dst_init=None):
+        s = [np.ascontiguousarray(p, dtype=np.int32) for p in src]
+        d = [p.copy() for p in (dst_init if dst_init is not None else s)]
+        c, g, keep = frame.cdef_args()
+        ss, ds = planes_struct(s), planes_struct(d)
+        self._st(self.fn["cdef"](C.byref(c), C.byref(g), ss, len(s), ds, len(d)), "cdef_frame")
+        return d
+
+    # upscale_frame (superres.rs:282): returns (status, outputs)
+    def upscale_frame(self, ctx, inp, out_shapes):
+        i = [np.ascontiguousarray(p, dtype=np.int32) for p in inp]
+        o = [np.zeros(s, dtype=np.int32) for s in out_shapes]
+        st = self.fn["sr"](C.byref(ctx), planes_struct(i), len(i), planes_struct(o), len(o))
+        return st, o
+
+    # loop_restoration_frame (loop_restoration.rs:268)
+    def loop_restoration_frame(self, frame, curr, cdef, ctx=None, grids=None, lr_init=None):
+        cu = [np.ascontiguousarray(p, dtype=np.int32) for p in curr]
+        cd = [np.ascontiguousarray(p, dtype=np.int32) for p in cdef]
+        lr = [p.copy() for p in lr_init] if lr_init is not None else [np.zeros_like(p) for p in cd]
+        if frame is not None:
+            c, g, keep = frame.lr_args()
+        if ctx is not None:
+            c = ctx
+        if grids is not None:
+            g = grids
+        self._st(self.fn["lr"](C.byref(c), C.byref(g), planes_struct(cu), len(cu),
+                               planes_struct(cd), len(cd), planes_struct(lr), len(lr)),
+                 "loop_restoration_frame")
+        return lr
+
+    # film_grain_synthesis (film_grain.rs:848), in place -> returns new arrays
+    def film_grain_synthesis(self, fg, bit_depth, num_planes, sx, sy, mc, planes):
+        out = [np.ascontiguousarray(p, dtype=np.int32).copy() for p in planes]
+        self._st(self.fn["fg"](C.byref(fg), bit_depth, num_planes, sx, sy, mc,
+                               planes_struct(out), len(out)), "film_grain_synthesis")
+        return out
+
+
+_gpu = None
+
+
+def gpu():
+    """FilterCalls over the HIP library (raises if it is not built)."""
+    global _gpu
+    if _gpu is None:
+        _gpu = FilterCalls(abi.load(), GPU_NAMES, status_returning=True)
+    return _gpu

@@ -1,0 +1,161 @@
+"""GPU parity of the five drop-in entry points against the CPU oracle.
+
+Each test generates a seeded frame, runs one stage through the HIP library
+(int32 PlaneBuffers staged to HBM) and through the oracle on the same
+inputs, and requires byte-identical planes.  The matrix covers bit depths
+8/10/12, subsampling 4:2:0/4:2:2/4:4:4/mono, odd frame sizes, superres
+denominators, restoration unit sizes and the film-grain options.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import gen_config
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = [  # (bit_depth, sx, sy, mono)
+    (8, 1, 1, 0), (10, 1, 1, 0), (12, 1, 1, 0), (8, 1, 0, 0), (10, 0, 0, 0), (12, 1, 0, 0),
+    (8, 1, 1, 1), (10, 0, 0, 0),
+]
+SIZES = [(96, 80), (100, 76), (64, 64), (200, 130)]
+
+
+def _frame(seed, **kw):
+    from oxav1 import Frame
+    return Frame.generate(gen_config(**kw), seed)
+
+
+def _eq(a_list, b_list, what):
+    assert len(a_list) == len(b_list)
+    for p, (a, b) in enumerate(zip(a_list, b_list)):
+        assert a.shape == b.shape, (what, p, a.shape, b.shape)
+        if not np.array_equal(a, b):
+            diff = np.argwhere(a != b)
+            y, x = diff[0]
+            raise AssertionError(f"{what}: plane {p}: {len(diff)} samples differ, first at "
+                                 f"({y},{x}): gpu {a[y, x]} oracle {b[y, x]}")
+
+
+def _fmt_kw(fmt, size):
+    bd, sx, sy, mono = fmt
+    return dict(upscaled_width=size[0], frame_height=size[1], bit_depth=bd, subsampling_x=sx,
+                subsampling_y=sy, mono=mono, noise_amp=8 << (bd - 8))
+
+
+@pytest.mark.parametrize("fi", range(len(FORMATS)))
+@pytest.mark.parametrize("si", range(len(SIZES)))
+def test_loop_filter_parity(fi, si, gpu_calls, oracle_mod):
+    seed = 0x1F00 + 17 * fi + si
+    f = _frame(seed, exercise_segments=si % 2, exercise_delta_lf=(si + fi) % 2,
+               **_fmt_kw(FORMATS[fi], SIZES[si]))
+    got = gpu_calls.loop_filter_frame(f, f.planes)
+    want = oracle_mod.calls().loop_filter_frame(f, f.planes)
+    assert any(not np.array_equal(w, p) for w, p in zip(want, f.planes)), "oracle filtered nothing"
+    _eq(got, want, "loop_filter_frame")
+
+
+@pytest.mark.parametrize("fi", range(len(FORMATS)))
+@pytest.mark.parametrize("si", range(len(SIZES)))
+def test_cdef_parity(fi, si, gpu_calls, oracle_mod):
+    seed = 0x2F00 + 17 * fi + si
+    f = _frame(seed, cdef_skip_units=1, **_fmt_kw(FORMATS[fi], SIZES[si]))
+    src = oracle_mod.calls().loop_filter_frame(f, f.planes)
+    got = gpu_calls.cdef_frame(f, src)
+    want = oracle_mod.calls().cdef_frame(f, src)
+    _eq(got, want, "cdef_frame")
+
+
+@pytest.mark.parametrize("denom", [9, 12, 16])
+@pytest.mark.parametrize("fi", [0, 1, 2, 3, 4, 6])
+def test_superres_parity(denom, fi, gpu_calls, oracle_mod):
+    seed = 0x3F00 + denom * 7 + fi
+    f = _frame(seed, superres_denom=denom, **_fmt_kw(FORMATS[fi], (150, 70)))
+    ctx = f.sr_ctx()
+    shapes = [f.out_shape(p) for p in range(f.num_planes)]
+    st_g, got = gpu_calls.upscale_frame(ctx, f.planes, shapes)
+    st_o, want = oracle_mod.calls().upscale_frame(ctx, f.planes, shapes)
+    assert st_g == st_o == 0
+    _eq(got, want, "upscale_frame")
+
+
+def test_superres_errors(gpu_calls, oracle_mod):
+    from oxav1 import abi
+    f = _frame(5, superres_denom=12, upscaled_width=96, frame_height=32)
+    ctx = f.sr_ctx()
+    shapes = [f.out_shape(p) for p in range(f.num_planes)]
+    # UpscaledNotLarger
+    bad = abi.SrCtx.from_buffer_copy(ctx)
+    bad.upscaled_width = bad.frame_width - 2
+    st_g, _ = gpu_calls.upscale_frame(bad, f.planes, shapes)
+    st_o, _ = oracle_mod.calls().upscale_frame(bad, f.planes, shapes)
+    assert st_g == st_o == abi.ERR_UPSCALED_NOT_LARGER
+    # shape mismatch on plane 1: plane 0 is still written (superres.rs:309-316)
+    shapes_bad = list(shapes)
+    shapes_bad[1] = (shapes[1][0], shapes[1][1] + 1)
+    st_g, got = gpu_calls.upscale_frame(ctx, f.planes, shapes_bad)
+    st_o, want = oracle_mod.calls().upscale_frame(ctx, f.planes, shapes_bad)
+    assert st_g == st_o == abi.ERR_PLANE_SHAPE_MISMATCH
+    assert np.array_equal(got[0], want[0]) and want[0].any()
+    # plane count
+    st_g, _ = gpu_calls.upscale_frame(ctx, f.planes[:1], shapes[:1])
+    st_o, _ = oracle_mod.calls().upscale_frame(ctx, f.planes[:1], shapes[:1])
+    assert st_g == st_o == abi.ERR_PLANE_COUNT_MISMATCH
+
+
+def _lr_inputs(f, oracle_mod):
+    o = oracle_mod.calls()
+    deb = o.loop_filter_frame(f, f.planes)
+    cdf = o.cdef_frame(f, deb)
+    return deb, cdf
+
+
+@pytest.mark.parametrize("fi", range(len(FORMATS)))
+@pytest.mark.parametrize("unit", [(0, 0), (1, 1), (2, 0)])
+def test_loop_restoration_parity(fi, unit, gpu_calls, oracle_mod):
+    seed = 0x4F00 + 31 * fi + unit[0] * 3 + unit[1]
+    f = _frame(seed, lr_unit_shift=unit[0], lr_uv_shift=unit[1],
+               **_fmt_kw(FORMATS[fi], (200, 150)))
+    deb, cdf = _lr_inputs(f, oracle_mod)
+    got = gpu_calls.loop_restoration_frame(f, deb, cdf)
+    want = oracle_mod.calls().loop_restoration_frame(f, deb, cdf)
+    _eq(got, want, "loop_restoration_frame")
+
+
+@pytest.mark.parametrize("types", [2, 4, 6])
+def test_loop_restoration_types(types, gpu_calls, oracle_mod):
+    f = _frame(0x4A00 + types, lr_types=types, bit_depth=10, noise_amp=64,
+               upscaled_width=260, frame_height=140)
+    deb, cdf = _lr_inputs(f, oracle_mod)
+    _eq(gpu_calls.loop_restoration_frame(f, deb, cdf),
+        oracle_mod.calls().loop_restoration_frame(f, deb, cdf), "loop_restoration_frame")
+
+
+@pytest.mark.parametrize("fi", [0, 1, 2, 3, 4, 6])
+@pytest.mark.parametrize("opts", [(0, 0, 8), (1, 3, 6), (1, 2, 0), (0, 1, 10)])
+def test_film_grain_parity(fi, opts, gpu_calls, oracle_mod):
+    overlap, lag, cpts = opts
+    seed = 0x5F00 + 13 * fi + lag
+    f = _frame(seed, enable_fg=1, fg_overlap=overlap, fg_ar_lag=lag, fg_num_y_points=10,
+               fg_chroma_points=cpts, **_fmt_kw(FORMATS[fi], (150, 98)))
+    d = f.desc
+    planes = [f.planes[p][: f.out_shape(p)[0], : f.out_shape(p)[1]].copy()
+              for p in range(f.num_planes)]
+    args = (d.fg, d.bit_depth, d.num_planes, d.subsampling_x, d.subsampling_y, d.matrix_coefficients)
+    got = gpu_calls.film_grain_synthesis(*args, planes)
+    want = oracle_mod.calls().film_grain_synthesis(*args, planes)
+    assert any(not np.array_equal(w, p) for w, p in zip(want, planes))
+    _eq(got, want, "film_grain_synthesis")
+
+
+def test_film_grain_chroma_from_luma(gpu_calls, oracle_mod):
+    f = _frame(0x5A11, enable_fg=1, fg_overlap=1, fg_ar_lag=3, fg_num_y_points=6,
+               upscaled_width=130, frame_height=70)
+    fg = f.desc.fg
+    fg.chroma_scaling_from_luma = 1
+    fg.num_cb_points = fg.num_cr_points = 0
+    planes = [f.planes[p][: f.out_shape(p)[0], : f.out_shape(p)[1]].copy() for p in range(3)]
+    args = (fg, 8, 3, 1, 1, 0)
+    _eq(gpu_calls.film_grain_synthesis(*args, planes),
+        oracle_mod.calls().film_grain_synthesis(*args, planes), "fg chroma_scaling_from_luma")
