@@ -292,6 +292,16 @@ int64_t oxav1_pf_table_checksum(const char *name);
  * unknown; lets foreign bindings verify their layouts. */
 size_t oxav1_pf_struct_size(const char *name);
 
+/* Device memory on the library's own HIP runtime, for callers that have no
+ * allocator of their own (bindings, tests, benchmarks).  Synchronous. */
+int oxav1_pf_dev_alloc(void **ptr, size_t bytes);
+int oxav1_pf_dev_free(void *ptr);
+int oxav1_pf_dev_upload(void *dst, const void *src, size_t bytes);
+int oxav1_pf_dev_download(void *dst, const void *src, size_t bytes);
+int oxav1_pf_dev_fill(void *dst, int byte_value, size_t bytes);
+/* Device properties: compute units and HBM bytes free / total. */
+int oxav1_pf_dev_info(int *num_cus, size_t *free_bytes, size_t *total_bytes);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident chain engine                                            */
 /* ---------------------------------------------------------------------- */

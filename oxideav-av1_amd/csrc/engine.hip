@@ -877,6 +877,51 @@ extern "C" int64_t oxav1_pf_table_checksum(const char *name) {
     return INT64_MIN;
 }
 
+extern "C" int oxav1_pf_dev_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return OXAV1_ERR_INVALID_ARG;
+    *ptr = nullptr;
+    int st = ensure_device_init();
+    if (st) return st;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_dev_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_dev_upload(void *dst, const void *src, size_t bytes) {
+    if (bytes && (!dst || !src)) return OXAV1_ERR_INVALID_ARG;
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_dev_download(void *dst, const void *src, size_t bytes) {
+    if (bytes && (!dst || !src)) return OXAV1_ERR_INVALID_ARG;
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_dev_fill(void *dst, int byte_value, size_t bytes) {
+    if (bytes && !dst) return OXAV1_ERR_INVALID_ARG;
+    if (bytes) {
+        HIP_TRY(hipMemset(dst, byte_value, bytes));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_dev_info(int *num_cus, size_t *free_bytes, size_t *total_bytes) {
+    int st = ensure_device_init();
+    if (st) return st;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return OXAV1_OK;
+}
+
 extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (!name) return 0;
     std::string n(name);

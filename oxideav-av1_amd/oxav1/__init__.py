@@ -5,7 +5,7 @@ this package only binds it with ctypes for tests, benchmarks and tools:
   abi      struct layouts and function signatures (checked against the C side)
   frames   seeded synthetic frames and drop-in argument bundles
   dropin   Python-level calls of the five drop-in entry points
-  engine   the device-resident chain engine on torch-allocated HBM buffers
+  engine   the device-resident chain engine and HBM buffers (library allocator)
 """
 from . import abi  # noqa: F401
 from .frames import Frame, planes_struct  # noqa: F401

@@ -188,6 +188,12 @@ _SIGS = {
     "oxav1_pf_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "oxav1_pf_engine_stage_ms": (C.c_int, [C.c_void_p, C.c_char_p, _P(C.c_double), _P(u32)]),
     "oxav1_pf_engine_stream": (C.c_void_p, [C.c_void_p]),
+    "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
+    "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "oxav1_pf_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "oxav1_pf_dev_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
+    "oxav1_pf_dev_info": (C.c_int, [_P(C.c_int), _P(C.c_size_t), _P(C.c_size_t)]),
     "oxav1_gen_default_config": (C.c_int, [C.c_int, u64, _P(GenConfig)]),
     "oxav1_gen_frame_create": (C.c_int, [_P(GenConfig), u64, _P(_P(GenFrame))]),
     "oxav1_gen_frame_destroy": (None, [_P(GenFrame)]),

@@ -1,7 +1,8 @@
-"""Device-resident chain engine on torch-allocated HBM buffers.
+"""Device-resident chain engine over HBM buffers.
 
-torch is used only to allocate device memory; every filter runs in the
-library's HIP kernels on the engine's own stream.
+Device memory comes from the library's own allocator (oxav1_pf_dev_*), so
+one HIP runtime serves the whole process; every filter runs in the
+library's HIP kernels on the engine's stream.
 """
 import ctypes as C
 
@@ -10,34 +11,70 @@ import numpy as np
 from . import abi
 
 
-def _torch():
-    import torch
-    return torch
+class DevBuffer:
+    """One device allocation holding a numpy-shaped array."""
+
+    def __init__(self, shape, dtype, data=None):
+        self.lib = abi.load()
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        abi.check(self.lib.oxav1_pf_dev_alloc(C.byref(p), self.nbytes), "dev_alloc")
+        self.ptr = p.value
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        abi.check(self.lib.oxav1_pf_dev_upload(self.ptr, a.ctypes.data, self.nbytes), "dev_upload")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        abi.check(self.lib.oxav1_pf_dev_download(out.ctypes.data, self.ptr, self.nbytes), "dev_download")
+        return out
+
+    def fill(self, byte_value=0):
+        abi.check(self.lib.oxav1_pf_dev_fill(self.ptr, byte_value, self.nbytes), "dev_fill")
+
+    def free(self):
+        if self.ptr:
+            self.lib.oxav1_pf_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_info():
+    lib = abi.load()
+    cus, free, total = C.c_int(), C.c_size_t(), C.c_size_t()
+    abi.check(lib.oxav1_pf_dev_info(C.byref(cus), C.byref(free), C.byref(total)), "dev_info")
+    return cus.value, free.value, total.value
 
 
 class DeviceFrame:
     """One frame's inputs and outputs in HBM, plus its FrameDesc/FrameDev."""
 
-    def __init__(self, frame, device="cuda", fg_out=True):
-        torch = _torch()
+    def __init__(self, frame, fg_out=True):
         self.frame = frame
         d = frame.desc
         self.desc = abi.FrameDesc.from_buffer_copy(d)
-        dt = torch.uint8 if d.bit_depth == 8 else torch.int16
-        self.sample_dtype = dt
+        st = np.uint8 if d.bit_depth == 8 else np.uint16
+        self.sample_dtype = st
         np_ = d.num_planes
-
-        def to_dev(a, dtype):
-            return torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)
-
-        self.src = [to_dev(frame.planes[p], dt) for p in range(np_)]
-        self.ref_out = [torch.zeros(frame.out_shape(p), dtype=dt, device=device) for p in range(np_)]
-        self.out = ([torch.zeros(frame.out_shape(p), dtype=dt, device=device) for p in range(np_)]
+        self.src = [DevBuffer(frame.planes[p].shape, st, frame.planes[p].astype(st)) for p in range(np_)]
+        self.ref_out = [DevBuffer(frame.out_shape(p), st) for p in range(np_)]
+        self.out = ([DevBuffer(frame.out_shape(p), st) for p in range(np_)]
                     if (fg_out and d.fg.apply_grain) else None)
-        self.lf_mi = to_dev(frame.lf_mi.view(np.int32), torch.int32)
-        self.delta_lf = to_dev(frame.delta_lf, torch.int8)
-        self.tx = [to_dev(frame.lf_tx_size[p], torch.uint8) for p in range(3)]
-        self.cdef_idx = to_dev(frame.cdef_idx, torch.int8)
+        self.lf_mi = DevBuffer(frame.lf_mi.shape, np.uint32, frame.lf_mi)
+        self.delta_lf = DevBuffer(frame.delta_lf.shape, np.int8, frame.delta_lf)
+        self.tx = [DevBuffer(frame.lf_tx_size[p].shape, np.uint8, frame.lf_tx_size[p]) for p in range(3)]
+        self.cdef_idx = DevBuffer(frame.cdef_idx.shape, np.int8, frame.cdef_idx)
         self.units = []
         for p in range(3):
             u = frame.lr_units[p]
@@ -45,32 +82,48 @@ class DeviceFrame:
                 self.units.append(None)
             else:
                 raw = np.frombuffer(bytes(u), dtype=np.uint8)
-                self.units.append(to_dev(raw, torch.uint8))
+                self.units.append(DevBuffer(raw.shape, np.uint8, raw))
         v = abi.FrameDev()
         for p in range(np_):
-            v.src[p] = self.src[p].data_ptr()
+            v.src[p] = self.src[p].ptr
             v.src_pitch[p] = self.src[p].shape[1]
-            v.ref_out[p] = self.ref_out[p].data_ptr()
+            v.ref_out[p] = self.ref_out[p].ptr
             v.ref_pitch[p] = self.ref_out[p].shape[1]
             if self.out is not None:
-                v.out[p] = self.out[p].data_ptr()
+                v.out[p] = self.out[p].ptr
                 v.out_pitch[p] = self.out[p].shape[1]
-        v.lf_mi = self.lf_mi.data_ptr()
-        v.delta_lf = self.delta_lf.data_ptr()
+        v.lf_mi = self.lf_mi.ptr
+        v.delta_lf = self.delta_lf.ptr
         for p in range(3):
-            v.lf_tx_size[p] = self.tx[p].data_ptr()
-            v.lr_units[p] = self.units[p].data_ptr() if self.units[p] is not None else None
-        v.cdef_idx = self.cdef_idx.data_ptr()
+            v.lf_tx_size[p] = self.tx[p].ptr
+            v.lr_units[p] = self.units[p].ptr if self.units[p] is not None else None
+        v.cdef_idx = self.cdef_idx.ptr
         self.dev = v
+
+    def clear_outputs(self):
+        for b in self.ref_out + (self.out or []):
+            b.fill(0)
 
     def result(self):
         """(ref_out, out) as numpy int32 planes (out = ref_out without grain)."""
-        ref = [t.cpu().numpy().astype(np.int32) for t in self.ref_out]
-        out = [t.cpu().numpy().astype(np.int32) for t in self.out] if self.out is not None else ref
+        ref = [b.download().astype(np.int32) for b in self.ref_out]
+        out = [b.download().astype(np.int32) for b in self.out] if self.out is not None else ref
         return ref, out
 
     def output_pixels(self):
         return int(self.desc.upscaled_width) * int(self.desc.frame_height)
+
+    def hbm_bytes(self):
+        """Bytes this frame keeps resident (inputs + outputs, not engine scratch)."""
+        bufs = self.src + self.ref_out + (self.out or []) + [self.lf_mi, self.delta_lf, self.cdef_idx]
+        bufs += self.tx + [u for u in self.units if u is not None]
+        return sum(b.nbytes for b in bufs)
+
+    def free(self):
+        bufs = self.src + self.ref_out + (self.out or []) + [self.lf_mi, self.delta_lf, self.cdef_idx]
+        bufs += self.tx + [u for u in self.units if u is not None]
+        for b in bufs:
+            b.free()
 
 
 class Engine:
@@ -95,7 +148,6 @@ class Engine:
             pass
 
     def prepare(self, frames):
-        _torch().cuda.synchronize()
         n = len(frames)
         descs = (abi.FrameDesc * n)(*[f.desc for f in frames])
         devs = (abi.FrameDev * n)(*[f.dev for f in frames])
@@ -117,6 +169,3 @@ class Engine:
         abi.check(self.lib.oxav1_pf_engine_stage_ms(self.h, stage.encode(), C.byref(ms), C.byref(cnt)),
                   "stage_ms")
         return ms.value, cnt.value
-
-    def stream_handle(self):
-        return self.lib.oxav1_pf_engine_stream(self.h)
