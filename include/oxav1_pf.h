@@ -301,6 +301,9 @@ int oxav1_pf_dev_download(void *dst, const void *src, size_t bytes);
 int oxav1_pf_dev_fill(void *dst, int byte_value, size_t bytes);
 /* Device properties: compute units and HBM bytes free / total. */
 int oxav1_pf_dev_info(int *num_cus, size_t *free_bytes, size_t *total_bytes);
+/* Device-to-device copy by a plain 16-byte-per-lane kernel (bytes % 16 == 0);
+ * used to calibrate HBM traffic counters against a known byte count. */
+int oxav1_pf_dev_copy_kernel(void *dst, const void *src, size_t bytes, int repeats);
 
 /* ---------------------------------------------------------------------- */
 /* Device-resident chain engine                                            */

@@ -32,48 +32,40 @@ struct CdefBlk {
     int16_t pri_y, sec_y, pri_uv, sec_uv;
 };
 
-// cdef_direction (cdef.rs:366-449) on an 8x8 block held in LDS.
+// cdef_direction (cdef.rs:366-449), costs of one half of the directions for
+// one 8x8 block read from the LDS tile: half 0 -> directions 0, 2, 4, 6;
+// half 1 -> directions 1, 3, 5, 7 (stored at cost[d]).  The block rows and
+// columns are clamped to the staged (buffer-bounded) extent lh x lw, as the
+// reference clamps its reads to the plane (cdef.rs:386-391).
 template <typename L>
-__device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *dir_out, int32_t *var_out) {
-    int32_t x[8][8];
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) x[i][j] = ((int32_t)t[i * stride + j] >> shift) - 128;
-    int32_t cost[8];
-    // directions 2 and 6
-    {
-        int32_t c2 = 0, c6 = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            int32_t s = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) s += x[i][j];
-            c2 += s * s;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            int32_t s = 0;
-#pragma unroll
-            for (int i = 0; i < 8; i++) s += x[i][j];
-            c6 += s * s;
-        }
-        cost[2] = c2 * 105;
-        cost[6] = c6 * 105;
-    }
-    // directions 0 and 4
-    {
-        int32_t p0[15], p4[15];
+__device__ void cdef_dir_costs(const L *t, int ls, int lh, int lw, int y8, int x8, int32_t shift, int half,
+                               int32_t *cost) {
+    auto px = [&](int i, int j) -> int32_t {
+        const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
+        return ((int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift) - 128;
+    };
+    if (half == 0) {
+        int32_t p0[15], p4[15], p2[8], p6[8];
 #pragma unroll
         for (int k = 0; k < 15; k++) p0[k] = p4[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) p2[k] = p6[k] = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                p0[i + j] += x[i][j];
-                p4[7 + i - j] += x[i][j];
+                const int32_t v = px(i, j);
+                p0[i + j] += v;
+                p4[7 + i - j] += v;
+                p2[i] += v;
+                p6[j] += v;
             }
-        int32_t c0 = 0, c4 = 0;
+        int32_t c2 = 0, c6 = 0, c0 = 0, c4 = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            c2 += p2[k] * p2[k];
+            c6 += p6[k] * p6[k];
+        }
 #pragma unroll
         for (int i = 0; i < 7; i++) {
             c0 += (p0[i] * p0[i] + p0[14 - i] * p0[14 - i]) * c_div_table[i + 1];
@@ -82,10 +74,10 @@ __device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *di
         c0 += p0[7] * p0[7] * 105;
         c4 += p4[7] * p4[7] * 105;
         cost[0] = c0;
+        cost[2] = c2 * 105;
         cost[4] = c4;
-    }
-    // directions 1, 3, 5, 7
-    {
+        cost[6] = c6 * 105;
+    } else {
         int32_t p1[11], p3[11], p5[11], p7[11];
 #pragma unroll
         for (int k = 0; k < 11; k++) p1[k] = p3[k] = p5[k] = p7[k] = 0;
@@ -93,10 +85,11 @@ __device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *di
         for (int i = 0; i < 8; i++)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                p1[i + j / 2] += x[i][j];
-                p3[3 + i - j / 2] += x[i][j];
-                p5[3 - i / 2 + j] += x[i][j];
-                p7[i / 2 + j] += x[i][j];
+                const int32_t v = px(i, j);
+                p1[i + j / 2] += v;
+                p3[3 + i - j / 2] += v;
+                p5[3 - i / 2 + j] += v;
+                p7[i / 2 + j] += v;
             }
         int32_t *ps[4] = {p1, p3, p5, p7};
 #pragma unroll
@@ -111,6 +104,10 @@ __device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *di
             cost[2 * d + 1] = c;
         }
     }
+}
+
+// yDir = first argmax (ties to the lowest direction), var (cdef.rs:434-447)
+__device__ __forceinline__ void cdef_pick_dir(const int32_t *cost, int32_t *dir_out, int32_t *var_out) {
     int32_t best = 0, dir = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++)
@@ -118,12 +115,8 @@ __device__ void cdef_find_dir(const L *t, int stride, int32_t shift, int32_t *di
             best = cost[k];
             dir = k;
         }
-    int32_t opp = cost[0];
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-        if (k == ((dir + 4) & 7)) opp = cost[k];
     *dir_out = dir;
-    *var_out = (best - opp) >> 10;
+    *var_out = (best - cost[(dir + 4) & 7]) >> 10;
 }
 
 // constrain (cdef.rs:592-606) with the per-block damping adjustment
@@ -211,7 +204,7 @@ __device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t 
 // One 64x64 unit (ur, uc in 64-luma units): deb -> cdf for all planes.
 template <typename T>
 __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
-                          CdefBlk *blk) {
+                          CdefBlk *blk, int32_t *costs /* 64 x 8 */) {
     using L = typename LdsOf<T>::type;
     const int tid = threadIdx.x;
     const uint32_t r0 = ur * 16, c0 = uc * 16;  // mi coordinates
@@ -232,31 +225,35 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
     L *t0 = lds;
     cdef_load<L, T>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
     __syncthreads();
-    // per 8x8 block: skip test + direction (one lane per block)
+    // per 8x8 block: skip test (cdef.rs:266-272, out-of-grid cells count as
+    // skip) and the direction search, split over two waves (half 0: directions
+    // 0/2/4/6, half 1: 1/3/5/7) that meet through `costs`
     const int nbx = (tw0 + 7) >> 3, nby = (th0 + 7) >> 3;
+    auto block_active = [&](int b) -> bool {
+        const int by = b >> 3, bx = b & 7;
+        if (!(by < nby && bx < nbx && do_cdef && idx >= 0)) return false;
+        const uint32_t r = r0 + 2 * by, c = c0 + 2 * bx;
+        if (r >= J.mi_rows || c >= J.mi_cols) return false;
+        auto skip_at = [&](uint32_t rr, uint32_t cc) -> bool {
+            return (J.lf_mi[(size_t)rr * J.mi_cols + cc] >> OXAV1_MI_SKIP_BIT) & 1u;
+        };
+        const bool skip = skip_at(r, c) && (r + 1 >= J.mi_rows || skip_at(r + 1, c)) &&
+                          (c + 1 >= J.mi_cols || skip_at(r, c + 1)) &&
+                          (r + 1 >= J.mi_rows || c + 1 >= J.mi_cols || skip_at(r + 1, c + 1));
+        return !skip;
+    };
+    if (tid < 128) {
+        const int b = tid & 63, half = tid >> 6;
+        if (block_active(b))
+            cdef_dir_costs<L>(t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, half, costs + b * 8);
+    }
+    __syncthreads();
     if (tid < 64) {
-        const int by = tid >> 3, bx = tid & 7;
         CdefBlk b{};
-        if (by < nby && bx < nbx && do_cdef && idx >= 0) {
-            const uint32_t r = r0 + 2 * by, c = c0 + 2 * bx;
-            auto skip_at = [&](uint32_t rr, uint32_t cc) -> bool {
-                return (J.lf_mi[(size_t)rr * J.mi_cols + cc] >> OXAV1_MI_SKIP_BIT) & 1u;
-            };
-            // cdef.rs:266-272: out-of-grid cells count as skip
-            const bool skip = skip_at(r, c) && (r + 1 >= J.mi_rows || skip_at(r + 1, c)) &&
-                              (c + 1 >= J.mi_cols || skip_at(r, c + 1)) &&
-                              (r + 1 >= J.mi_rows || c + 1 >= J.mi_cols || skip_at(r + 1, c + 1));
-            if (r < J.mi_rows && c < J.mi_cols && !skip) {
-                // direction over the 8x8 luma block, reads clamped to the buffer (cdef.rs:386-391)
-                L blkv[64];
-                for (int i = 0; i < 8; i++)
-                    for (int j = 0; j < 8; j++) {
-                        const int32_t yy = min(8 * by + i, lh0 - 1);
-                        const int32_t xx = min(8 * bx + j, lw0 - 1);
-                        blkv[i * 8 + j] = t0[(yy + kCdefB) * ls0 + xx + kCdefB];
-                    }
+        if (block_active(tid)) {
+            {
                 int32_t ydir, var;
-                cdef_find_dir<L>(blkv, 8, coeff_shift, &ydir, &var);
+                cdef_pick_dir(costs + tid * 8, &ydir, &var);
                 const int32_t pri_y = (int32_t)J.cdef.y_pri_strength[idx] << coeff_shift;
                 const int32_t sec_y = (int32_t)J.cdef.y_sec_strength[idx] << coeff_shift;
                 const int32_t var_str = (var >> 6) > 0 ? min((int32_t)floor_log2((uint32_t)(var >> 6)), 12) : 0;

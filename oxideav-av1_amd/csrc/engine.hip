@@ -2,8 +2,9 @@
 //
 // Stage kernels take a table of FrameJobs.  Each launch processes work items
 // (frame, plane, tile): plain launches cover one frame with one workgroup per
-// item; batched launches are persistent (about two workgroups per CU) and
-// pull items from a device counter, so frames of any size, bit depth and
+// item; batched launches are persistent (as many workgroups as the kernel's
+// occupancy allows on every CU) and pull items from a device counter, so
+// frames of any size, bit depth and
 // subsampling share one launch per stage.  The batched stage sequence can be
 // captured once in a hipGraph and replayed.
 //
@@ -75,25 +76,37 @@ __device__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::ty
 }
 
 template <typename T>
-__device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk) {
+__device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
+                          int32_t *costs) {
     const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
-    cdef_unit<T>(J, ur, uc, tile, blk);
+    cdef_unit<T>(J, ur, uc, tile, blk, costs);
 }
 
+// Superres items: planes 0..2 upscale the CDEF frame in runs of
+// sr_rows_per_item rows; planes 3..5 upscale the deblocked frame only on the
+// four rows around each stripe boundary k >= 1: rows k*(64>>sy) - (8>>sy) - 2
+// .. +1 (the rows loop restoration reads from it).
 template <typename T>
-__device__ void sr_item(const FrameJob &J, uint32_t local) {
+__device__ void sr_item(const FrameJob &J, uint32_t local, SrSmem<typename LdsOf<T>::type> &S) {
     const uint32_t p6 = plane_of(J.item_base[kStageSr], 6, local);
     const uint32_t chunk = local - J.item_base[kStageSr][p6];
     const uint32_t p = p6 % 3;
     const bool deb = p6 >= 3;
-    const int32_t y0 = (int32_t)(chunk * J.sr_rows_per_item);
-    const int32_t y1 = min(y0 + (int32_t)J.sr_rows_per_item, (int32_t)J.out_rows[p]);
+    int32_t y0, nrows;
+    if (!deb) {
+        y0 = (int32_t)(chunk * J.sr_rows_per_item);
+        nrows = (int32_t)J.sr_rows_per_item;
+    } else {
+        const int32_t sy = p ? J.sy : 0;
+        y0 = (int32_t)(chunk + 1) * (64 >> sy) - (8 >> sy) - 2;
+        nrows = 4;
+    }
     const T *in = plane_ptr<T>(deb ? (const void *)J.deb[p] : (const void *)J.cdf[p]);
     const uint32_t ip = deb ? J.deb_pitch[p] : J.cdf_pitch[p];
     T *out = plane_ptr<T>(deb ? J.up_deb[p] : J.up_cdef[p]);
     const uint32_t op = deb ? J.up_deb_pitch[p] : J.up_cdef_pitch[p];
-    sr_rows<T>(in, ip, (int32_t)J.pad_cols[p], out, op, (int32_t)J.out_cols[p], y0, y1, J.sr_step[p], J.sr_init[p],
-               J.sr_maxx[p], J.bd);
+    sr_rows<T>(in, ip, (int32_t)J.pad_cols[p], out, op, (int32_t)J.out_cols[p], y0, nrows, (int32_t)J.out_rows[p],
+               J.sr_step[p], J.sr_init[p], J.sr_maxx[p], J.bd, S);
 }
 
 template <typename T>
@@ -162,14 +175,15 @@ __global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const u
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L tile[kCdefLw * kCdefLw];
     __shared__ CdefBlk blk[64];
+    __shared__ int32_t costs[64 * 8];
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            cdef_item<int32_t>(J, ref.local, tile, blk);
+            cdef_item<int32_t>(J, ref.local, tile, blk, costs);
         } else {
             if (J.kind == kU8)
-                cdef_item<uint8_t>(J, ref.local, tile, blk);
+                cdef_item<uint8_t>(J, ref.local, tile, blk, costs);
             else
-                cdef_item<uint16_t>(J, ref.local, tile, blk);
+                cdef_item<uint16_t>(J, ref.local, tile, blk, costs);
         }
     })
 }
@@ -177,14 +191,16 @@ __global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const u
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_sr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
+    using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
+    __shared__ SrSmem<L> S;
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            sr_item<int32_t>(J, ref.local);
+            sr_item<int32_t>(J, ref.local, S);
         } else {
             if (J.kind == kU8)
-                sr_item<uint8_t>(J, ref.local);
+                sr_item<uint8_t>(J, ref.local, S);
             else
-                sr_item<uint16_t>(J, ref.local);
+                sr_item<uint16_t>(J, ref.local, S);
         }
     })
 }
@@ -364,6 +380,8 @@ struct FrameSlot {
 struct oxav1_pf_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;       // film-grain template branch
+    hipEvent_t fork = nullptr, join = nullptr;
     bool own_stream = false;
     uint32_t max_frames = 0;
     int num_cus = 0;
@@ -380,7 +398,7 @@ struct oxav1_pf_engine {
     hipEvent_t ev[8] = {};
     double stage_sum[6] = {0, 0, 0, 0, 0, 0};
     uint32_t stage_cnt[6] = {0, 0, 0, 0, 0, 0};
-    int persistent_blocks = 0;
+    int persistent_blocks[5] = {0, 0, 0, 0, 0};
 };
 
 static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
@@ -561,7 +579,11 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         J.item_base[kStageSr][p6] = acc;
         const uint32_t p = p6 % 3;
         const bool on = J.use_sr && p < np && (p6 < 3 || J.sr_deb_on);
-        if (on) acc += ceil_div(J.out_rows[p], J.sr_rows_per_item);
+        if (on && p6 < 3) acc += ceil_div(J.out_rows[p], J.sr_rows_per_item);
+        if (on && p6 >= 3) {  // stripe-boundary row groups below the last output row
+            const uint32_t sy = p ? J.sy : 0;
+            acc += (J.out_rows[p] - 1 + (8u >> sy) + 2) / (64u >> sy);
+        }
     }
     J.item_base[kStageSr][6] = acc;
     acc = 0;
@@ -606,7 +628,18 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) e->num_cus = prop.multiProcessorCount;
     if (e->num_cus <= 0) e->num_cus = 256;
-    e->persistent_blocks = e->num_cus * 2;
+    // persistent grid per stage: every workgroup slot the kernel can occupy
+    {
+        const void *ks[5] = {(const void *)k_lf<0>, (const void *)k_cdef<0>, (const void *)k_sr<0>,
+                             (const void *)k_lr<0>, (const void *)k_fg<0>};
+        for (int s = 0; s < 5; s++) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[s], kThreads, 0) != hipSuccess ||
+                per_cu <= 0)
+                per_cu = 2;
+            e->persistent_blocks[s] = e->num_cus * per_cu;
+        }
+    }
     e->max_frames = max_frames ? max_frames : 1;
     if (hip_stream) {
         e->stream = (hipStream_t)hip_stream;
@@ -622,6 +655,12 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
             delete e;
             return OXAV1_ERR_HIP;
         }
+    if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join, hipEventDisableTiming) != hipSuccess) {
+        delete e;
+        return OXAV1_ERR_HIP;
+    }
     st = e->counters_d.ensure(8 * sizeof(uint32_t));
     if (st) {
         delete e;
@@ -642,6 +681,9 @@ extern "C" int oxav1_pf_engine_destroy(oxav1_pf_engine *e) {
     e->counters_d.release();
     for (auto &ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->join) (void)hipEventDestroy(e->join);
+    if (e->side) (void)hipStreamDestroy(e->side);
     if (e->own_stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return OXAV1_OK;
@@ -694,12 +736,16 @@ static int enqueue_plain(oxav1_pf_engine *e, bool timed) {
     const FrameJob *jd = (const FrameJob *)e->jobs_d.p;
     for (uint32_t f = 0; f < e->n; f++) {
         const FrameJob &J = e->jobs_h[f];
-        if (J.apply_fg) {
-            hipLaunchKernelGGL(k_fg_prep, dim3(1), dim3(kThreads), 0, e->stream, jd + f, 1u);
+        if (J.apply_fg) {  // grain templates on the side branch, joined before the blend
+            HIP_TRY(hipEventRecord(e->fork, e->stream));
+            HIP_TRY(hipStreamWaitEvent(e->side, e->fork, 0));
+            hipLaunchKernelGGL(k_fg_prep, dim3(1), dim3(kThreads), 0, e->side, jd + f, 1u);
             HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(e->join, e->side));
         }
         if (timed && f == 0) HIP_TRY(hipEventRecord(e->ev[0], e->stream));
         for (int s = 0; s < 5; s++) {
+            if (s == kStageFg && J.apply_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
             const uint32_t items = stage_items(J, s);
             if (items) {
                 hipLaunchKernelGGL(kEngineKernels[s], dim3(items), dim3(kThreads), 0, e->stream, jd + f,
@@ -719,15 +765,21 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
     const uint32_t *pre = (const uint32_t *)e->prefix_d.p;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, e->stream, ctr, 8u);
     HIP_TRY(hipGetLastError());
+    // Grain templates depend only on the parameters: build them on a side
+    // branch that overlaps deblock..restoration, joined before the blend.
     if (e->any_fg) {
-        hipLaunchKernelGGL(k_fg_prep, dim3(e->n), dim3(kThreads), 0, e->stream, jd, e->n);
+        HIP_TRY(hipEventRecord(e->fork, e->stream));
+        HIP_TRY(hipStreamWaitEvent(e->side, e->fork, 0));
+        hipLaunchKernelGGL(k_fg_prep, dim3(e->n), dim3(kThreads), 0, e->side, jd, e->n);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(e->join, e->side));
     }
     if (timed) HIP_TRY(hipEventRecord(e->ev[0], e->stream));
     for (int s = 0; s < 5; s++) {
         const uint32_t total = e->totals[s];
+        if (s == kStageFg && e->any_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
         if (total) {
-            const uint32_t grid = std::min<uint32_t>(total, (uint32_t)e->persistent_blocks);
+            const uint32_t grid = std::min<uint32_t>(total, (uint32_t)e->persistent_blocks[s]);
             hipLaunchKernelGGL(kEngineKernels[s], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
                                e->n, total, ctr + s);
             HIP_TRY(hipGetLastError());
@@ -919,6 +971,25 @@ extern "C" int oxav1_pf_dev_info(int *num_cus, size_t *free_bytes, size_t *total
     HIP_TRY(hipMemGetInfo(&f, &t));
     if (free_bytes) *free_bytes = f;
     if (total_bytes) *total_bytes = t;
+    return OXAV1_OK;
+}
+
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int oxav1_pf_dev_copy_kernel(void *dst, const void *src, size_t bytes, int repeats) {
+    if (!dst || !src || bytes % 16) return OXAV1_ERR_INVALID_ARG;
+    int st = ensure_device_init();
+    if (st) return st;
+    const size_t n = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    for (int r = 0; r < repeats; r++) {
+        hipLaunchKernelGGL(k_copy16, dim3(grid ? grid : 1), dim3(256), 0, nullptr, (uint4 *)dst, (const uint4 *)src, n);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
     return OXAV1_OK;
 }
 

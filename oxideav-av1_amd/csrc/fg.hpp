@@ -94,6 +94,76 @@ __device__ void fg_white(int16_t *g, int count, uint32_t seed, bool on, uint32_t
     }
 }
 
+// Causal auto-regressive filter of NT templates of width w, height h in LDS
+// (film_grain.rs:255-289, 319-372).  Wavefront: lane j filters row 3+j and at
+// step t reaches column 3+t-4j, so row y-1 finished column x+3 one step
+// earlier.  Each lane slides the LAG rows above (columns x-LAG..x+LAG) and
+// the last LAG outputs of its own row through registers; per step it reads
+// one new column of the rows above.  Coefficient order follows the
+// reference's pos counter: rows -LAG..-1 left to right, then row 0 columns
+// -LAG..-1, then (for chroma) the luma-average tap.
+template <int LAG, int NT, typename LumaTerm>
+__device__ void fg_ar(int16_t *const *g, int w, int h, const int32_t (*cf)[25], uint32_t ash, int32_t gmin,
+                      int32_t gmax, LumaTerm luma) {
+    constexpr int K = 2 * LAG + 1;
+    constexpr int R = LAG > 0 ? LAG : 1;
+    const int rows = h - 3, cols = w - 6;
+    const int steps = cols + 4 * (rows - 1);
+    const int j = threadIdx.x;
+    const int y = 3 + j;
+    int32_t win[NT][R][K];
+    int32_t cur[NT][R];
+    for (int t = 0; t < steps; t++) {
+        const int x = 3 + t - 4 * j;
+        if (j < rows && x >= 3 && x < w - 3) {
+            if constexpr (LAG > 0) {
+                if (x == 3) {
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+#pragma unroll
+                        for (int r = 0; r < LAG; r++)
+#pragma unroll
+                            for (int k = 0; k < K; k++) win[n][r][k] = g[n][(y - LAG + r) * w + (x - LAG + k)];
+#pragma unroll
+                        for (int k = 0; k < LAG; k++) cur[n][k] = g[n][y * w + (x - LAG + k)];
+                    }
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+#pragma unroll
+                        for (int r = 0; r < LAG; r++) {
+#pragma unroll
+                            for (int k = 0; k < K - 1; k++) win[n][r][k] = win[n][r][k + 1];
+                            win[n][r][K - 1] = g[n][(y - LAG + r) * w + (x + LAG)];
+                        }
+                }
+            }
+            const int32_t lt = luma(0, x, y);
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                int32_t s = 0;
+                if constexpr (LAG > 0) {
+#pragma unroll
+                    for (int r = 0; r < LAG; r++)
+#pragma unroll
+                        for (int k = 0; k < K; k++) s += win[n][r][k] * cf[n][r * K + k];
+#pragma unroll
+                    for (int k = 0; k < LAG; k++) s += cur[n][k] * cf[n][LAG * K + k];
+                }
+                s += lt * cf[n][LAG * K + LAG];
+                const int32_t v = clip3(gmin, gmax, (int32_t)g[n][y * w + x] + fg_round2(s, ash));
+                g[n][y * w + x] = (int16_t)v;
+                if constexpr (LAG > 0) {
+#pragma unroll
+                    for (int k = 0; k < LAG - 1; k++) cur[n][k] = cur[n][k + 1];
+                    cur[n][LAG - 1] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // One workgroup per frame (blockDim.x >= 128).
 __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) {
     FgScratch *S = J.fgs;
@@ -139,71 +209,47 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
     fg_white(Cb, cw * ch, fg.grain_seed ^ 0xb524u, chroma_on_cb, shift);
     fg_white(Cr, cw * ch, fg.grain_seed ^ 0x49d8u, chroma_on_cr, shift);
     __syncthreads();
-    const int32_t lag = fg.ar_coeff_lag;
+    const int lag = fg.ar_coeff_lag;
     const uint32_t ash = fg.ar_coeff_shift;
-    // luma AR wavefront: lane j owns row 3+j; at step t it filters column 3+t-4j
+    // luma AR (film_grain.rs:255-289)
     if (fg.num_y_points > 0) {
-        const int rows = kGrainH - 3, colsn = kGrainW - 6;
-        const int steps = colsn + 4 * (rows - 1);
-        for (int t = 0; t < steps; t++) {
-            const int j = tid;
-            if (j < rows) {
-                const int x = 3 + t - 4 * j;
-                if (x >= 3 && x < kGrainW - 3) {
-                    const int y = 3 + j;
-                    int32_t s = 0;
-                    int pos = 0;
-                    for (int dr = -lag; dr <= 0; dr++)
-                        for (int dc = -lag; dc <= lag; dc++) {
-                            if (dr == 0 && dc == 0) goto done_l;
-                            s += (int32_t)L[(y + dr) * kGrainW + (x + dc)] * ((int32_t)fg.ar_coeffs_y_plus_128[pos] - 128);
-                            pos++;
-                        }
-                done_l:
-                    L[y * kGrainW + x] = (int16_t)clip3(gmin, gmax, (int32_t)L[y * kGrainW + x] + fg_round2(s, ash));
-                }
-            }
-            __syncthreads();
+        int16_t *gs[1] = {L};
+        int32_t cf[1][25];
+        for (int i = 0; i < 24; i++) cf[0][i] = (int32_t)fg.ar_coeffs_y_plus_128[i] - 128;
+        cf[0][24] = 0;
+        auto none = [](int, int, int) { return 0; };
+        switch (lag) {
+        case 0: fg_ar<0, 1>(gs, kGrainW, kGrainH, cf, ash, gmin, gmax, none); break;
+        case 1: fg_ar<1, 1>(gs, kGrainW, kGrainH, cf, ash, gmin, gmax, none); break;
+        case 2: fg_ar<2, 1>(gs, kGrainW, kGrainH, cf, ash, gmin, gmax, none); break;
+        default: fg_ar<3, 1>(gs, kGrainW, kGrainH, cf, ash, gmin, gmax, none); break;
         }
     }
-    // chroma AR (Cb and Cr together), film_grain.rs:319-372
+    __syncthreads();
+    // chroma AR, Cb and Cr together (film_grain.rs:319-372); the centre tap
+    // adds the co-located luma average when num_y_points > 0
     if (fg.chroma_scaling_from_luma || fg.num_cb_points > 0 || fg.num_cr_points > 0) {
-        const int rows = ch - 3, colsn = cw - 6;
-        const int steps = colsn + 4 * (rows - 1);
-        for (int t = 0; t < steps; t++) {
-            const int j = tid;
-            if (j < rows) {
-                const int x = 3 + t - 4 * j;
-                if (x >= 3 && x < cw - 3) {
-                    const int y = 3 + j;
-                    int32_t s0 = 0, s1 = 0;
-                    int pos = 0;
-                    for (int dr = -lag; dr <= 0; dr++)
-                        for (int dc = -lag; dc <= lag; dc++) {
-                            const int32_t c0 = (int32_t)fg.ar_coeffs_cb_plus_128[pos] - 128;
-                            const int32_t c1 = (int32_t)fg.ar_coeffs_cr_plus_128[pos] - 128;
-                            if (dr == 0 && dc == 0) {
-                                if (fg.num_y_points > 0) {
-                                    const int lx = ((x - 3) << sx) + 3, ly = ((y - 3) << sy) + 3;
-                                    int32_t l = 0;
-                                    for (int i = 0; i <= (int)sy; i++)
-                                        for (int k = 0; k <= (int)sx; k++) l += (int32_t)L[(ly + i) * kGrainW + (lx + k)];
-                                    l = fg_round2(l, sx + sy);
-                                    s0 += l * c0;
-                                    s1 += l * c1;
-                                }
-                                goto done_c;
-                            }
-                            s0 += (int32_t)Cb[(y + dr) * cw + (x + dc)] * c0;
-                            s1 += (int32_t)Cr[(y + dr) * cw + (x + dc)] * c1;
-                            pos++;
-                        }
-                done_c:
-                    Cb[y * cw + x] = (int16_t)clip3(gmin, gmax, (int32_t)Cb[y * cw + x] + fg_round2(s0, ash));
-                    Cr[y * cw + x] = (int16_t)clip3(gmin, gmax, (int32_t)Cr[y * cw + x] + fg_round2(s1, ash));
-                }
-            }
-            __syncthreads();
+        int16_t *gs[2] = {Cb, Cr};
+        int32_t cf[2][25];
+        for (int i = 0; i < 25; i++) {
+            cf[0][i] = (int32_t)fg.ar_coeffs_cb_plus_128[i] - 128;
+            cf[1][i] = (int32_t)fg.ar_coeffs_cr_plus_128[i] - 128;
+        }
+        const bool with_luma = fg.num_y_points > 0;
+        const int isx = (int)sx, isy = (int)sy;
+        auto luma = [=](int n, int x, int y) -> int32_t {
+            if (!with_luma) return 0;
+            const int lx = ((x - 3) << isx) + 3, ly = ((y - 3) << isy) + 3;
+            int32_t l = 0;
+            for (int i = 0; i <= isy; i++)
+                for (int k = 0; k <= isx; k++) l += (int32_t)L[(ly + i) * kGrainW + (lx + k)];
+            return fg_round2(l, (uint32_t)(isx + isy));
+        };
+        switch (lag) {
+        case 0: fg_ar<0, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
+        case 1: fg_ar<1, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
+        case 2: fg_ar<2, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
+        default: fg_ar<3, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
         }
     }
     __syncthreads();

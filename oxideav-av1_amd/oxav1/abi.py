@@ -194,6 +194,7 @@ _SIGS = {
     "oxav1_pf_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "oxav1_pf_dev_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "oxav1_pf_dev_info": (C.c_int, [_P(C.c_int), _P(C.c_size_t), _P(C.c_size_t)]),
+    "oxav1_pf_dev_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "oxav1_gen_default_config": (C.c_int, [C.c_int, u64, _P(GenConfig)]),
     "oxav1_gen_frame_create": (C.c_int, [_P(GenConfig), u64, _P(_P(GenFrame))]),
     "oxav1_gen_frame_destroy": (None, [_P(GenFrame)]),
