@@ -38,8 +38,8 @@ struct CdefBlk {
 // columns are clamped to the staged (buffer-bounded) extent lh x lw, as the
 // reference clamps its reads to the plane (cdef.rs:386-391).
 template <typename L>
-__device__ void cdef_dir_costs(const L *t, int ls, int lh, int lw, int y8, int x8, int32_t shift, int half,
-                               int32_t *cost) {
+__device__ void cdef_dir_costs(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift, int half,
+                               S3<int32_t> *cost) {
     auto px = [&](int i, int j) -> int32_t {
         const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
         return ((int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift) - 128;
@@ -136,7 +136,7 @@ __device__ __forceinline__ int32_t cdef_adj(int32_t damping, int32_t thr) {
 template <typename L, typename T, bool CHECK>
 __device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
                                   int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefBlk *blk, int blk_stride,
-                                  bool chroma, int32_t damping, int32_t coeff_shift, T *__restrict__ dst,
+                                  bool chroma, int32_t damping, int32_t coeff_shift, G<T> *__restrict__ dst,
                                   uint32_t dpitch) {
     for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
         const int yy = i / tw, xx = i - yy * tw;
@@ -190,7 +190,7 @@ __device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox
 
 template <typename L, typename T>
 __device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
-                          const T *__restrict__ src, uint32_t spitch) {
+                          const G<T> *__restrict__ src, uint32_t spitch) {
     const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
     for (int i = threadIdx.x; i < ls * lh; i += blockDim.x) {
         const int yy = i / ls, xx = i - yy * ls;
@@ -209,7 +209,7 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
     const int tid = threadIdx.x;
     const uint32_t r0 = ur * 16, c0 = uc * 16;  // mi coordinates
     const int32_t coeff_shift = (int32_t)J.bd - 8;
-    const int8_t idx = J.cdef_idx[(size_t)ur * J.cdef_ucols + uc];
+    const int8_t idx = gread(J.cdef_idx, (size_t)ur * J.cdef_ucols + uc);
     const bool do_cdef = J.apply_cdef != 0;
     // luma tile
     // The 8x8 walk writes min(buffer, walked extent) (cdef.rs:185-200, 320-352);
@@ -235,7 +235,7 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
         const uint32_t r = r0 + 2 * by, c = c0 + 2 * bx;
         if (r >= J.mi_rows || c >= J.mi_cols) return false;
         auto skip_at = [&](uint32_t rr, uint32_t cc) -> bool {
-            return (J.lf_mi[(size_t)rr * J.mi_cols + cc] >> OXAV1_MI_SKIP_BIT) & 1u;
+            return (gread(J.lf_mi, (size_t)rr * J.mi_cols + cc) >> OXAV1_MI_SKIP_BIT) & 1u;
         };
         const bool skip = skip_at(r, c) && (r + 1 >= J.mi_rows || skip_at(r + 1, c)) &&
                           (c + 1 >= J.mi_cols || skip_at(r, c + 1)) &&
@@ -245,7 +245,8 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
     if (tid < 128) {
         const int b = tid & 63, half = tid >> 6;
         if (block_active(b))
-            cdef_dir_costs<L>(t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, half, costs + b * 8);
+            cdef_dir_costs<L>((const S3<L> *)t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, half,
+                             (S3<int32_t> *)(costs + b * 8));
     }
     __syncthreads();
     if (tid < 64) {
@@ -274,7 +275,7 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
     {
         const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
         const int32_t damping = (int32_t)J.cdef.cdef_damping + coeff_shift;
-        T *dst = plane_ptr<T>(J.cdf[0]);
+        G<T> *dst = plane_ptr<T>(J.cdf[0]);
         if (inside)
             cdef_filter_plane<L, T, false>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, blk, 8, false, damping,
                                            coeff_shift, dst, J.cdf_pitch[0]);
@@ -298,7 +299,7 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
         __syncthreads();
         const int ls = lw + 2 * kCdefB;
         const bool inside = ox >= kCdefB && oy >= kCdefB && ox + tw + kCdefB <= pw && oy + th + kCdefB <= ph;
-        T *dst = plane_ptr<T>(J.cdf[p]);
+        G<T> *dst = plane_ptr<T>(J.cdf[p]);
         if (inside)
             cdef_filter_plane<L, T, false>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, blk, 8, true, damping_uv,
                                            coeff_shift, dst, J.cdf_pitch[p]);

@@ -119,13 +119,30 @@ __device__ __forceinline__ uint32_t floor_log2(uint32_t x) {
     return x == 0 ? 0u : 31u - (uint32_t)__clz((int)x);
 }
 
+// Pointers in the FrameJob are generic (flat) addresses; the frame buffers
+// all live in device global memory, so casting them to address space 1 lets
+// the compiler emit global_load/global_store instead of flat accesses (which
+// also tie up the LDS counter and cannot use the scalar-offset forms).
 template <typename T>
-__device__ __forceinline__ T *plane_ptr(void *p) {
-    return static_cast<T *>(p);
+using G = __attribute__((address_space(1))) T;
+// LDS (address space 3), for helpers that are not inlined into the kernel.
+template <typename T>
+using S3 = __attribute__((address_space(3))) T;
+
+// Element i of a side-information array the FrameJob points at, read as a
+// global load.
+template <typename T>
+__device__ __forceinline__ T gread(const T *p, size_t i) {
+    return ((const G<T> *)p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ G<T> *plane_ptr(void *p) {
+    return (G<T> *)p;
 }
 template <typename T>
-__device__ __forceinline__ const T *plane_ptr(const void *p) {
-    return static_cast<const T *>(p);
+__device__ __forceinline__ const G<T> *plane_ptr(const void *p) {
+    return (const G<T> *)p;
 }
 
 }  // namespace oxav1

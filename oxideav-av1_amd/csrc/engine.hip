@@ -87,7 +87,7 @@ __device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::
 // four rows around each stripe boundary k >= 1: rows k*(64>>sy) - (8>>sy) - 2
 // .. +1 (the rows loop restoration reads from it).
 template <typename T>
-__device__ void sr_item(const FrameJob &J, uint32_t local, SrSmem<typename LdsOf<T>::type> &S) {
+__device__ void sr_item(const FrameJob &J, uint32_t local) {
     const uint32_t p6 = plane_of(J.item_base[kStageSr], 6, local);
     const uint32_t chunk = local - J.item_base[kStageSr][p6];
     const uint32_t p = p6 % 3;
@@ -101,12 +101,12 @@ __device__ void sr_item(const FrameJob &J, uint32_t local, SrSmem<typename LdsOf
         y0 = (int32_t)(chunk + 1) * (64 >> sy) - (8 >> sy) - 2;
         nrows = 4;
     }
-    const T *in = plane_ptr<T>(deb ? (const void *)J.deb[p] : (const void *)J.cdf[p]);
+    const G<T> *in = plane_ptr<T>(deb ? (const void *)J.deb[p] : (const void *)J.cdf[p]);
     const uint32_t ip = deb ? J.deb_pitch[p] : J.cdf_pitch[p];
-    T *out = plane_ptr<T>(deb ? J.up_deb[p] : J.up_cdef[p]);
+    G<T> *out = plane_ptr<T>(deb ? J.up_deb[p] : J.up_cdef[p]);
     const uint32_t op = deb ? J.up_deb_pitch[p] : J.up_cdef_pitch[p];
     sr_rows<T>(in, ip, (int32_t)J.pad_cols[p], out, op, (int32_t)J.out_cols[p], y0, nrows, (int32_t)J.out_rows[p],
-               J.sr_step[p], J.sr_init[p], J.sr_maxx[p], J.bd, S);
+               J.sr_step[p], J.sr_init[p], J.sr_maxx[p], J.bd);
 }
 
 template <typename T>
@@ -122,7 +122,7 @@ __device__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
     const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
     const uint32_t t = local - J.item_base[kStageFg][p];
     const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
-    for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = J.fgs->lut[i >> 8][i & 255];
+    for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)J.fgs)->lut[i >> 8][i & 255];
     __syncthreads();
     const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
     const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
@@ -191,16 +191,14 @@ __global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const u
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_sr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
-    using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
-    __shared__ SrSmem<L> S;
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            sr_item<int32_t>(J, ref.local, S);
+            sr_item<int32_t>(J, ref.local);
         } else {
             if (J.kind == kU8)
-                sr_item<uint8_t>(J, ref.local, S);
+                sr_item<uint8_t>(J, ref.local);
             else
-                sr_item<uint16_t>(J, ref.local, S);
+                sr_item<uint16_t>(J, ref.local);
         }
     })
 }

@@ -57,7 +57,7 @@ __device__ __forceinline__ uint32_t fg_jump(uint32_t r, uint32_t n) {
 __device__ __forceinline__ int32_t fg_round2(int32_t x, uint32_t n) { return n == 0 ? x : (x + (1 << (n - 1))) >> n; }
 
 // scaling_lookup_fill (film_grain.rs:480-526), one lane per plane
-__device__ void fg_lut_fill(uint8_t *t, uint32_t np_, const uint8_t *xs, const uint8_t *ys) {
+__device__ void fg_lut_fill(G<uint8_t> *t, uint32_t np_, const uint8_t *xs, const uint8_t *ys) {
     if (np_ == 0) {
         for (int v = 0; v < 256; v++) t[v] = 0;
         return;
@@ -166,7 +166,7 @@ __device__ void fg_ar(int16_t *const *g, int w, int h, const int32_t (*cf)[25], 
 
 // One workgroup per frame (blockDim.x >= 128).
 __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) {
-    FgScratch *S = J.fgs;
+    G<FgScratch> *S = (G<FgScratch> *)J.fgs;
     const oxav1_film_grain_params &fg = J.fg;
     const int tid = threadIdx.x;
     const uint32_t bd = J.bd, sx = J.sx, sy = J.sy;
@@ -178,7 +178,7 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
     const uint32_t np_ = J.np < 3 ? J.np : 3;
     // scaling LUTs (scaling_lookup_init, film_grain.rs:418-466)
     if (tid < 3) {
-        uint8_t *t = S->lut[tid];
+        G<uint8_t> *t = S->lut[tid];
         if ((uint32_t)tid >= np_)
             for (int v = 0; v < 256; v++) t[v] = 0;
         else if (tid == 0 || fg.chroma_scaling_from_luma)
@@ -195,7 +195,7 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
         r ^= (s * 173u + 105u) & 255u;
         for (uint32_t b = 0; b < S->nblocks; b++) {
             fg_step(r);
-            S->offsets[s * S->nblocks + b] = (uint8_t)((r >> 8) & 255u);
+            ((G<uint8_t> *)S->offsets)[s * S->nblocks + b] = (uint8_t)((r >> 8) & 255u);
         }
     }
     if (tid == 0) {
@@ -261,7 +261,7 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
 }
 
 // Grain sample of (stripe s, block b, row i, col j) for a plane.
-__device__ __forceinline__ int32_t fg_grain(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+__device__ __forceinline__ int32_t fg_grain(const G<int16_t> *tmpl, int tw, const G<uint8_t> *offs, uint32_t nblocks,
                                             uint32_t s, uint32_t b, int32_t i, int32_t j, uint32_t psx, uint32_t psy) {
     const uint32_t rnd = offs[s * nblocks + b];
     const int32_t ox = (int32_t)(rnd >> 4), oy = (int32_t)(rnd & 15u);
@@ -271,7 +271,7 @@ __device__ __forceinline__ int32_t fg_grain(const int16_t *tmpl, int tw, const u
 }
 
 // Value of noiseStripe[s][plane][i][x] after the horizontal overlap blend.
-__device__ __forceinline__ int32_t fg_hval(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+__device__ __forceinline__ int32_t fg_hval(const G<int16_t> *tmpl, int tw, const G<uint8_t> *offs, uint32_t nblocks,
                                            uint32_t s, int32_t i, int32_t x, uint32_t psx, uint32_t psy, bool overlap,
                                            int32_t gmin, int32_t gmax) {
     if (!psx) {
@@ -296,7 +296,7 @@ __device__ __forceinline__ int32_t fg_hval(const int16_t *tmpl, int tw, const ui
 }
 
 // noiseImage[plane][y][x] (film_grain.rs:705-739)
-__device__ __forceinline__ int32_t fg_noise(const int16_t *tmpl, int tw, const uint8_t *offs, uint32_t nblocks,
+__device__ __forceinline__ int32_t fg_noise(const G<int16_t> *tmpl, int tw, const G<uint8_t> *offs, uint32_t nblocks,
                                             int32_t y, int32_t x, uint32_t psx, uint32_t psy, bool overlap,
                                             int32_t gmin, int32_t gmax) {
     const uint32_t s = (uint32_t)y >> (5 - psy);
@@ -329,7 +329,7 @@ __device__ __forceinline__ int32_t fg_scale(const uint8_t *t, int32_t index, uin
 template <typename T>
 __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
                               const uint8_t *lut /* LDS [3][256] */) {
-    const FgScratch *S = J.fgs;
+    const G<FgScratch> *S = (const G<FgScratch> *)J.fgs;
     const oxav1_film_grain_params &fg = J.fg;
     const uint32_t bd = J.bd;
     const uint32_t psx = plane ? J.sx : 0, psy = plane ? J.sy : 0;
@@ -346,13 +346,13 @@ __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int
     }
     const uint32_t gs = fg.grain_scaling;
     const bool overlap = fg.overlap_flag != 0;
-    const T *in = plane_ptr<T>(J.ref_out[plane]);
-    T *out = plane_ptr<T>(J.out[plane]);
+    const G<T> *in = plane_ptr<T>(J.ref_out[plane]);
+    G<T> *out = plane_ptr<T>(J.out[plane]);
     const uint32_t ip = J.ref_pitch[plane], op = J.out_pitch[plane];
     const int32_t w = (int32_t)J.out_cols[0];
     const int32_t cols = x1 - x0;
     bool active;
-    const int16_t *tmpl;
+    const G<int16_t> *tmpl;
     int tw;
     if (plane == 0) {
         active = fg.num_y_points > 0;
@@ -366,7 +366,7 @@ __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int
         tmpl = plane == 1 ? S->cb : S->cr;
         tw = S->chroma_w;
     }
-    const T *ly = plane_ptr<T>(J.ref_out[0]);
+    const G<T> *ly = plane_ptr<T>(J.ref_out[0]);
     const uint32_t lp = J.ref_pitch[0];
     for (int32_t i = threadIdx.x; i < (y1 - y0) * cols; i += blockDim.x) {
         const int32_t y = y0 + i / cols, x = x0 + i % cols;
@@ -375,7 +375,7 @@ __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int
             out[(size_t)y * op + x] = (T)orig;
             continue;
         }
-        const int32_t nz = fg_noise(tmpl, tw, S->offsets, S->nblocks, y, x, psx, psy, overlap, gmin, gmax);
+        const int32_t nz = fg_noise(tmpl, tw, (const G<uint8_t> *)S->offsets, S->nblocks, y, x, psx, psy, overlap, gmin, gmax);
         if (plane == 0) {
             const int32_t n2 = fg_round2(fg_scale(lut, orig, bd) * nz, gs);
             out[(size_t)y * op + x] = (T)clip3(minv, maxl, orig + n2);

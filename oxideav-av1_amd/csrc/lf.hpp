@@ -40,14 +40,14 @@ __constant__ uint8_t c_subsampled[88];  // filled from kSubsampledSize at init
 __device__ __forceinline__ uint32_t lf_level_at(const FrameJob &J, uint32_t row, uint32_t col,
                                                 uint32_t plane, uint32_t pass) {
     const size_t k = (size_t)row * J.mi_cols + col;
-    const uint32_t w = J.lf_mi[k];
+    const uint32_t w = gread(J.lf_mi, k);
     const uint32_t seg = w >> OXAV1_MI_SEG_SHIFT;
     const uint32_t ref = (w >> OXAV1_MI_REF_SHIFT) & 15u;
     const uint32_t mode = (w >> OXAV1_MI_MODE_SHIFT) & 31u;
     const uint32_t mode_type = (mode >= 14 && mode != 16 && mode != 24) ? 1u : 0u;
     const uint32_t i = plane == 0 ? pass : plane + 1;
     const uint32_t dli = J.lf_delta_multi ? i : 0u;
-    const int32_t dlf = J.delta_lf ? (int32_t)J.delta_lf[k * 4 + dli] : 0;
+    const int32_t dlf = J.delta_lf ? (int32_t)gread(J.delta_lf, k * 4 + dli) : 0;
     int32_t lvl = clip3(0, 63, dlf + (int32_t)J.lf_level[i]);
     const uint32_t feature = 1 + i;
     if (seg < 8 && J.seg_active[seg][feature]) lvl = clip3(0, 63, (int32_t)J.seg_data[seg][feature] + lvl);
@@ -86,11 +86,11 @@ __device__ uint32_t lf_decide(const FrameJob &J, uint32_t plane, uint32_t pass, 
     const int32_t prev_col = (int32_t)col - ((pass == 0 ? 1 : 0) << sx);
     if (prev_row < 0 || prev_col < 0) return 0;
     if (row >= J.mi_rows || col >= J.mi_cols) return 0;
-    const uint32_t w = J.lf_mi[(size_t)row * J.mi_cols + col];
+    const uint32_t w = gread(J.lf_mi, (size_t)row * J.mi_cols + col);
     const uint32_t mi_size = (w >> OXAV1_MI_SIZE_SHIFT) & 31u;
     const uint32_t tcols = J.tx_cols[plane];
-    const uint32_t tx_sz = J.tx[plane][(size_t)(row >> sy) * tcols + (col >> sx)];
-    const uint32_t ptx = J.tx[plane][(size_t)((uint32_t)prev_row >> sy) * tcols + ((uint32_t)prev_col >> sx)];
+    const uint32_t tx_sz = gread(J.tx[plane], (size_t)(row >> sy) * tcols + (col >> sx));
+    const uint32_t ptx = gread(J.tx[plane], (size_t)((uint32_t)prev_row >> sy) * tcols + ((uint32_t)prev_col >> sx));
     // planeSize = get_plane_residual_size(MiSize, plane).unwrap_or(MiSize)
     uint32_t plane_size = mi_size;
     if (mi_size < 22) {
@@ -242,7 +242,7 @@ __device__ void lf_filter_sample(const LfRegion<L> &R, int32_t x, int32_t y, int
 // One 64x64 tile of one plane: src (T) -> dst (T).
 template <typename T>
 __device__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t tx,
-                        const T *__restrict__ src, uint32_t spitch, T *__restrict__ dst,
+                        const G<T> *__restrict__ src, uint32_t spitch, G<T> *__restrict__ dst,
                         uint32_t dpitch, typename LdsOf<T>::type *sm, uint16_t *dec0, uint16_t *dec1) {
     using L = typename LdsOf<T>::type;
     const int32_t rows = (int32_t)J.pad_rows[plane], cols = (int32_t)J.pad_cols[plane];

@@ -57,12 +57,12 @@ struct LrTileGeom {
 
 // get_source_sample (loop_restoration.rs:862-902) for a virtual (x, y).
 template <typename T>
-__device__ __forceinline__ int32_t lr_fetch(const LrTileGeom &g, const T *__restrict__ cur, uint32_t cpitch,
-                                            int32_t crows, int32_t ccols, const T *__restrict__ cdf,
+__device__ __forceinline__ int32_t lr_fetch(const LrTileGeom &g, const G<T> *__restrict__ cur, uint32_t cpitch,
+                                            int32_t crows, int32_t ccols, const G<T> *__restrict__ cdf,
                                             uint32_t dpitch, int32_t drows, int32_t dcols, int32_t x, int32_t y) {
     x = clip3(0, max(g.pex, 0), x);
     y = clip3(0, max(g.pey, 0), y);
-    const T *b;
+    const G<T> *b;
     uint32_t pitch;
     int32_t rows, cols;
     if (y < g.ss) {
@@ -111,7 +111,7 @@ __device__ __forceinline__ void lr_ab(const L *src, int32_t wy, int32_t wx, int3
 // equal plane coordinates).  Writes rows [ys, ye] x cols [xs, xe].
 template <typename T>
 __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t tile,
-                        LrSmem<typename LdsOf<T>::type> &S, T *__restrict__ out, uint32_t opitch) {
+                        LrSmem<typename LdsOf<T>::type> &S, G<T> *__restrict__ out, uint32_t opitch) {
     using L = typename LdsOf<T>::type;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
     LrTileGeom g;
@@ -125,8 +125,8 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
     g.xs = (int32_t)tile * tw;
     g.xe = min(g.xs + tw - 1, g.pex);
     if (g.ys > g.ye || g.xs > g.xe) return;
-    const T *cur = plane_ptr<T>(J.up_deb[plane]);
-    const T *cdf = plane_ptr<T>(J.up_cdef[plane]);
+    const G<T> *cur = plane_ptr<T>(J.up_deb[plane]);
+    const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
     const uint32_t cpitch = J.up_deb_pitch[plane], dpitch = J.up_cdef_pitch[plane];
     const int32_t rows = (int32_t)J.lr_rows[plane], cols = (int32_t)J.lr_cols[plane];
     const int32_t crows = (int32_t)J.lrc_rows[plane], ccols = (int32_t)J.lrc_cols[plane];
