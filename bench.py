@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2, help="frames timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="timed batched steps only (for rocprofv3)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run only the plain device copy kernel over this many bytes, --steps times "
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters against a known traffic)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -57,6 +60,22 @@ def main():
     lib = abi.load()
     abi.check(lib.oxav1_pf_init(), "init")
     num_cus, _, hbm_total = device_info()
+
+    if args.copy_bytes:
+        from oxav1.engine import DevBuffer
+        n = (args.copy_bytes + 15) // 16 * 16
+        src = DevBuffer((n,), np.uint8)
+        dst = DevBuffer((n,), np.uint8)
+        src.fill(1)
+        t = time.perf_counter()
+        abi.check(lib.oxav1_pf_dev_copy_kernel(dst.ptr, src.ptr, n, args.steps), "copy kernel")
+        s = (time.perf_counter() - t) / max(args.steps, 1)
+        print(json.dumps({"copy_kernel_bytes": n, "launches": args.steps,
+                          "read_bytes_per_launch": n, "write_bytes_per_launch": n,
+                          "approx_gbs": round(2 * n / s / 1e9, 1)}))
+        src.free()
+        dst.free()
+        return
 
     base = 0x0A1F0000 + args.cfg
     seeds = [int(lib.oxav1_gen_splitmix64(base ^ i)) for i in range(args.frames)]

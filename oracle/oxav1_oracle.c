@@ -60,6 +60,16 @@ static inline void pb_set(oxav1_plane_i32 *b, int32_t y, int32_t x, int32_t v) {
     b->samples[(size_t)y * b->cols + (size_t)x] = v;
 }
 
+/* Deblocking reads go through lf_get.  oo_loop_filter_frame_parallel points
+ * s_lf_read at a snapshot of the plane taken at the start of each pass, so
+ * every edge of the pass reads the pass-start samples (all edges "at once"),
+ * while writes still land in the live plane.  Test infrastructure for the
+ * sequential-vs-parallel equivalence of the in-place walk. */
+static const oxav1_plane_i32 *s_lf_read = NULL;
+static inline int32_t lf_get(const oxav1_plane_i32 *b, int32_t y, int32_t x) {
+    return pb_get(s_lf_read ? s_lf_read : b, y, x);
+}
+
 /* ===================================================================== */
 /* cdf.rs helpers                                                        */
 /* ===================================================================== */
@@ -186,14 +196,14 @@ static inline int32_t filter4_clamp(int32_t v, uint8_t bd) {
 oo_lf_masks oo_lf_filter_mask(const oxav1_plane_i32 *b, int32_t x, int32_t y, uint8_t plane,
                               uint8_t limit, uint8_t blimit, uint8_t thresh, int32_t dx,
                               int32_t dy, uint32_t filter_size, uint8_t bit_depth) {
-    int32_t q0 = pb_get(b, y, x), q1 = pb_get(b, y + dy, x + dx);
-    int32_t q2 = pb_get(b, y + 2 * dy, x + 2 * dx), q3 = pb_get(b, y + 3 * dy, x + 3 * dx);
-    int32_t q4 = pb_get(b, y + 4 * dy, x + 4 * dx), q5 = pb_get(b, y + 5 * dy, x + 5 * dx);
-    int32_t q6 = pb_get(b, y + 6 * dy, x + 6 * dx);
-    int32_t p0 = pb_get(b, y - dy, x - dx), p1 = pb_get(b, y - 2 * dy, x - 2 * dx);
-    int32_t p2 = pb_get(b, y - 3 * dy, x - 3 * dx), p3 = pb_get(b, y - 4 * dy, x - 4 * dx);
-    int32_t p4 = pb_get(b, y - 5 * dy, x - 5 * dx), p5 = pb_get(b, y - 6 * dy, x - 6 * dx);
-    int32_t p6 = pb_get(b, y - 7 * dy, x - 7 * dx);
+    int32_t q0 = lf_get(b, y, x), q1 = lf_get(b, y + dy, x + dx);
+    int32_t q2 = lf_get(b, y + 2 * dy, x + 2 * dx), q3 = lf_get(b, y + 3 * dy, x + 3 * dx);
+    int32_t q4 = lf_get(b, y + 4 * dy, x + 4 * dx), q5 = lf_get(b, y + 5 * dy, x + 5 * dx);
+    int32_t q6 = lf_get(b, y + 6 * dy, x + 6 * dx);
+    int32_t p0 = lf_get(b, y - dy, x - dx), p1 = lf_get(b, y - 2 * dy, x - 2 * dx);
+    int32_t p2 = lf_get(b, y - 3 * dy, x - 3 * dx), p3 = lf_get(b, y - 4 * dy, x - 4 * dx);
+    int32_t p4 = lf_get(b, y - 5 * dy, x - 5 * dx), p5 = lf_get(b, y - 6 * dy, x - 6 * dx);
+    int32_t p6 = lf_get(b, y - 7 * dy, x - 7 * dx);
     uint32_t bds = (uint32_t)bit_depth - 8;
     int32_t thresh_bd = (int32_t)thresh << bds;
     oo_lf_masks m;
@@ -244,8 +254,8 @@ oo_lf_masks oo_lf_filter_mask(const oxav1_plane_i32 *b, int32_t x, int32_t y, ui
 void oo_lf_narrow_filter(oxav1_plane_i32 *b, int32_t x, int32_t y, int32_t dx, int32_t dy,
                          int hev_mask, uint8_t bd) {
     int32_t half = 0x80 << (bd - 8);
-    int32_t q0 = pb_get(b, y, x), q1 = pb_get(b, y + dy, x + dx);
-    int32_t p0 = pb_get(b, y - dy, x - dx), p1 = pb_get(b, y - 2 * dy, x - 2 * dx);
+    int32_t q0 = lf_get(b, y, x), q1 = lf_get(b, y + dy, x + dx);
+    int32_t p0 = lf_get(b, y - dy, x - dx), p1 = lf_get(b, y - 2 * dy, x - 2 * dx);
     int32_t ps1 = p1 - half, ps0 = p0 - half, qs0 = q0 - half, qs1 = q1 - half;
     int32_t filter = hev_mask ? filter4_clamp(ps1 - qs1, bd) : 0;
     filter = filter4_clamp(filter + 3 * (qs0 - ps0), bd);
@@ -276,7 +286,7 @@ void oo_lf_wide_filter(oxav1_plane_i32 *b, int32_t x, int32_t y, uint8_t plane, 
         for (int32_t j = -n; j <= n; j++) {
             int32_t p = clip3(-(n + 1), n, i + j);
             int64_t tap = iabs(j) <= n2 ? 2 : 1;
-            t += (int64_t)pb_get(b, y + p * dy, x + p * dx) * tap;
+            t += (int64_t)lf_get(b, y + p * dy, x + p * dx) * tap;
         }
         f[idx] = (int32_t)round2_i64(t, log2_size);
     }
@@ -372,6 +382,31 @@ void oo_loop_filter_frame(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
             for (uint32_t row = 0; row < ctx->mi_rows; row += row_step)
                 for (uint32_t col = 0; col < ctx->mi_cols; col += col_step)
                     oo_loop_filter_edge(ctx, g, planes, n, (uint8_t)plane, pass, row, col);
+    }
+}
+
+/* The walk of oo_loop_filter_frame with every edge of a (plane, pass)
+ * reading the samples as they were when the pass began. */
+void oo_loop_filter_frame_parallel(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                                   oxav1_plane_i32 *planes, uint32_t n) {
+    uint32_t np = ctx->num_planes < n ? ctx->num_planes : n;
+    for (uint32_t plane = 0; plane < np; plane++) {
+        if (plane > 0 && ctx->loop_filter_level[1 + plane] == 0) continue;
+        uint32_t sx = plane ? ctx->subsampling_x : 0, sy = plane ? ctx->subsampling_y : 0;
+        uint32_t row_step = plane == 0 ? 1 : (1u << sy);
+        uint32_t col_step = plane == 0 ? 1 : (1u << sx);
+        size_t cnt = (size_t)planes[plane].rows * planes[plane].cols;
+        oxav1_plane_i32 snap = planes[plane];
+        snap.samples = (int32_t *)malloc(cnt * sizeof(int32_t) + 4);
+        for (uint8_t pass = 0; pass < 2; pass++) {
+            memcpy(snap.samples, planes[plane].samples, cnt * sizeof(int32_t));
+            s_lf_read = &snap;
+            for (uint32_t row = 0; row < ctx->mi_rows; row += row_step)
+                for (uint32_t col = 0; col < ctx->mi_cols; col += col_step)
+                    oo_loop_filter_edge(ctx, g, planes, n, (uint8_t)plane, pass, row, col);
+            s_lf_read = NULL;
+        }
+        free(snap.samples);
     }
 }
 

@@ -35,6 +35,9 @@ typedef struct oo_lf_masks {
 
 void oo_loop_filter_frame(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
                           oxav1_plane_i32 *planes, uint32_t n);
+/* Test helper: every edge of a pass reads the pass-start samples. */
+void oo_loop_filter_frame_parallel(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                                   oxav1_plane_i32 *planes, uint32_t n);
 void oo_loop_filter_edge(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
                          oxav1_plane_i32 *planes, uint32_t n, uint8_t plane, uint8_t pass,
                          uint32_t row, uint32_t col);

@@ -131,69 +131,105 @@ __device__ __forceinline__ int32_t cdef_adj(int32_t damping, int32_t thr) {
     return min(max(0, damping - (int32_t)floor_log2((uint32_t)thr)), 31);
 }
 
-// Filter one plane of a unit from an LDS tile with a kCdefB border.
+// Taps of one 8x8 block for the plane being filtered, resolved once per
+// block: LDS offsets of the six "+" taps (primary k=0,1; secondary dir+2 and
+// dir-2 at k=0,1; the "-" taps are the negated offsets), their (dy, dx) for
+// the per-tap availability test, strengths and damping shifts.
+struct CdefTaps {
+    int16_t off[6];
+    int16_t pri, sec;
+    uint8_t pri_adj, sec_adj, pt0, pt1;
+    uint8_t active, pad[3];
+    int8_t dy[6], dx[6];
+};
+
+__device__ __forceinline__ CdefTaps cdef_taps(const CdefBlk &b, bool chroma, int32_t damping,
+                                             int32_t coeff_shift, int ls) {
+    CdefTaps t{};
+    const int32_t pri = chroma ? b.pri_uv : b.pri_y;
+    const int32_t sec = chroma ? b.sec_uv : b.sec_y;
+    const int32_t dir = chroma ? b.dir_uv : b.dir_y;
+    t.active = b.active;
+    t.pri = (int16_t)pri;
+    t.sec = (int16_t)sec;
+    t.pri_adj = (uint8_t)cdef_adj(damping, pri);
+    t.sec_adj = (uint8_t)cdef_adj(damping, sec);
+    const int pri_row = (pri >> coeff_shift) & 1;  // Cdef_Pri_Taps[(priStr >> coeffShift) & 1]
+    t.pt0 = pri_row ? 3 : 4;
+    t.pt1 = pri_row ? 3 : 2;
+    const int32_t ds[6] = {dir, dir, (dir + 2) & 7, (dir - 2) & 7, (dir + 2) & 7, (dir - 2) & 7};
+    const int ks[6] = {0, 1, 0, 0, 1, 1};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int32_t dy = c_cdef_dir[ds[i]][ks[i]][0], dx = c_cdef_dir[ds[i]][ks[i]][1];
+        t.dy[i] = (int8_t)dy;
+        t.dx[i] = (int8_t)dx;
+        t.off[i] = (int16_t)(dy * ls + dx);
+    }
+    return t;
+}
+
+// Filter one plane of a unit from an LDS tile with a kCdefB border
+// (cdef_filter_block, cdef.rs:465-590).  Lanes own columns: 1 << row_log2
+// consecutive threads cover one tile row, and each thread walks down its
+// column in steps of blockDim.x >> row_log2 rows.
 // CHECK: per-tap "inside the plane" test (units touching the frame edge).
 template <typename L, typename T, bool CHECK>
 __device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
-                                  int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefBlk *blk, int blk_stride,
-                                  bool chroma, int32_t damping, int32_t coeff_shift, G<T> *__restrict__ dst,
-                                  uint32_t dpitch) {
-    for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
-        const int yy = i / tw, xx = i - yy * tw;
-        const int32_t py = oy + yy, px = ox + xx;
-        if (py >= ph || px >= pw) continue;
-        const CdefBlk &b = blk[(yy >> bh_log2) * blk_stride + (xx >> bw_log2)];
+                                  int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
+                                  int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
+    const int xx = threadIdx.x & ((1 << row_log2) - 1);
+    const int ystep = blockDim.x >> row_log2;
+    if (xx >= tw) return;
+    const int32_t px = ox + xx;
+    if (px >= pw) return;
+    for (int yy = threadIdx.x >> row_log2; yy < th; yy += ystep) {
+        const int32_t py = oy + yy;
+        if (py >= ph) break;
+        const CdefTaps b = taps[(yy >> bh_log2) * 8 + (xx >> bw_log2)];
         const L *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
         const int32_t x = (int32_t)c[0];
-        if (!b.active) {
-            dst[(size_t)py * dpitch + px] = (T)x;
-            continue;
-        }
-        const int32_t pri = chroma ? b.pri_uv : b.pri_y;
-        const int32_t sec = chroma ? b.sec_uv : b.sec_y;
-        const int32_t dir = chroma ? b.dir_uv : b.dir_y;
-        const int32_t pri_adj = cdef_adj(damping, pri), sec_adj = cdef_adj(damping, sec);
-        const int pri_row = (pri >> coeff_shift) & 1;
-        const int32_t pt0 = pri_row ? 3 : 4, pt1 = pri_row ? 3 : 2;
-        int32_t sum = 0, mx = x, mn = x;
+        int32_t out = x;
+        if (b.active) {
+            int32_t sum = 0, mx = x, mn = x;
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
+            for (int i = 0; i < 6; i++) {
+                const int32_t thr = i < 2 ? b.pri : b.sec;
+                const int32_t adj = i < 2 ? b.pri_adj : b.sec_adj;
+                const int32_t w = i == 0 ? b.pt0 : (i == 1 ? b.pt1 : (i < 4 ? 2 : 1));
 #pragma unroll
-            for (int si = 0; si < 2; si++) {
-                const int32_t sgn = si ? 1 : -1;
-                {
-                    const int32_t dy = sgn * c_cdef_dir[dir][k][0], dx = sgn * c_cdef_dir[dir][k][1];
-                    if (!CHECK || (py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) {
-                        const int32_t v = (int32_t)c[dy * ls + dx];
-                        sum += (k ? pt1 : pt0) * cdef_constrain(v - x, pri, pri_adj);
-                        mx = max(mx, v);
-                        mn = min(mn, v);
+                for (int sg = 0; sg < 2; sg++) {
+                    if (CHECK) {
+                        const int32_t dy = sg ? -b.dy[i] : b.dy[i], dx = sg ? -b.dx[i] : b.dx[i];
+                        if (!(py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) continue;
                     }
-                }
-#pragma unroll
-                for (int oi = 0; oi < 2; oi++) {
-                    const int32_t d2 = (dir + (oi ? 2 : -2)) & 7;
-                    const int32_t dy = sgn * c_cdef_dir[d2][k][0], dx = sgn * c_cdef_dir[d2][k][1];
-                    if (!CHECK || (py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) {
-                        const int32_t v = (int32_t)c[dy * ls + dx];
-                        sum += (k ? 1 : 2) * cdef_constrain(v - x, sec, sec_adj);
-                        mx = max(mx, v);
-                        mn = min(mn, v);
-                    }
+                    const int32_t v = (int32_t)c[sg ? -b.off[i] : b.off[i]];
+                    sum += w * cdef_constrain(v - x, thr, adj);
+                    mx = max(mx, v);
+                    mn = min(mn, v);
                 }
             }
+            const int32_t delta = (8 + sum - (sum < 0 ? 1 : 0)) >> 4;
+            out = clip3(mn, mx, x + delta);
         }
-        const int32_t delta = (8 + sum - (sum < 0 ? 1 : 0)) >> 4;
-        dst[(size_t)py * dpitch + px] = (T)clip3(mn, mx, x + delta);
+        dst[(size_t)py * dpitch + px] = (T)out;
     }
+}
+
+// Exact i / d for 0 <= i < 2^20 / d with m = ceil(2^20 / d), d <= 128.
+__device__ __forceinline__ int div_magic(int i, uint32_t m) {
+    return (int)(((uint32_t)i * m) >> 20);
 }
 
 template <typename L, typename T>
 __device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
                           const G<T> *__restrict__ src, uint32_t spitch) {
     const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
-    for (int i = threadIdx.x; i < ls * lh; i += blockDim.x) {
-        const int yy = i / ls, xx = i - yy * ls;
+    const uint32_t m = ((1u << 20) + (uint32_t)ls - 1) / (uint32_t)ls;
+    const int n = ls * lh;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int yy = div_magic(i, m), xx = i - yy * ls;
         const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
         // Out-of-plane entries are never used as taps; clamp keeps the load in bounds.
         const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
@@ -204,7 +240,7 @@ __device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t 
 // One 64x64 unit (ur, uc in 64-luma units): deb -> cdf for all planes.
 template <typename T>
 __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
-                          CdefBlk *blk, int32_t *costs /* 64 x 8 */) {
+                          CdefBlk *blk, CdefTaps *taps, int32_t *costs /* 64 x 8 */) {
     using L = typename LdsOf<T>::type;
     const int tid = threadIdx.x;
     const uint32_t r0 = ur * 16, c0 = uc * 16;  // mi coordinates
@@ -273,15 +309,15 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
     __syncthreads();
     // luma filter
     {
-        const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
         const int32_t damping = (int32_t)J.cdef.cdef_damping + coeff_shift;
+        if (tid < 64) taps[tid] = cdef_taps(blk[tid], false, damping, coeff_shift, ls0);
+        __syncthreads();
+        const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
         G<T> *dst = plane_ptr<T>(J.cdf[0]);
         if (inside)
-            cdef_filter_plane<L, T, false>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, blk, 8, false, damping,
-                                           coeff_shift, dst, J.cdf_pitch[0]);
+            cdef_filter_plane<L, T, false>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, taps, 6, dst, J.cdf_pitch[0]);
         else
-            cdef_filter_plane<L, T, true>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, blk, 8, false, damping,
-                                          coeff_shift, dst, J.cdf_pitch[0]);
+            cdef_filter_plane<L, T, true>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, taps, 6, dst, J.cdf_pitch[0]);
     }
     if (J.np < 2) return;
     // chroma planes (cdef.rs:296-313), each staged in turn
@@ -296,16 +332,17 @@ __device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename 
         __syncthreads();
         if (tw <= 0 || th <= 0) continue;
         cdef_load<L, T>(t0, lw, lh, ox, oy, pw, ph, plane_ptr<T>(J.deb[p]), J.deb_pitch[p]);
-        __syncthreads();
         const int ls = lw + 2 * kCdefB;
+        if (p == 1 && tid < 64) taps[tid] = cdef_taps(blk[tid], true, damping_uv, coeff_shift, ls);
+        __syncthreads();
         const bool inside = ox >= kCdefB && oy >= kCdefB && ox + tw + kCdefB <= pw && oy + th + kCdefB <= ph;
         G<T> *dst = plane_ptr<T>(J.cdf[p]);
         if (inside)
-            cdef_filter_plane<L, T, false>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, blk, 8, true, damping_uv,
-                                           coeff_shift, dst, J.cdf_pitch[p]);
+            cdef_filter_plane<L, T, false>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, taps, 6 - sx, dst,
+                                           J.cdf_pitch[p]);
         else
-            cdef_filter_plane<L, T, true>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, blk, 8, true, damping_uv,
-                                          coeff_shift, dst, J.cdf_pitch[p]);
+            cdef_filter_plane<L, T, true>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, taps, 6 - sx, dst,
+                                          J.cdf_pitch[p]);
     }
 }
 

@@ -77,9 +77,9 @@ __device__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::ty
 
 template <typename T>
 __device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
-                          int32_t *costs) {
+                          CdefTaps *taps, int32_t *costs) {
     const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
-    cdef_unit<T>(J, ur, uc, tile, blk, costs);
+    cdef_unit<T>(J, ur, uc, tile, blk, taps, costs);
 }
 
 // Superres items: planes 0..2 upscale the CDEF frame in runs of
@@ -175,15 +175,16 @@ __global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const u
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L tile[kCdefLw * kCdefLw];
     __shared__ CdefBlk blk[64];
+    __shared__ CdefTaps taps[64];
     __shared__ int32_t costs[64 * 8];
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            cdef_item<int32_t>(J, ref.local, tile, blk, costs);
+            cdef_item<int32_t>(J, ref.local, tile, blk, taps, costs);
         } else {
             if (J.kind == kU8)
-                cdef_item<uint8_t>(J, ref.local, tile, blk, costs);
+                cdef_item<uint8_t>(J, ref.local, tile, blk, taps, costs);
             else
-                cdef_item<uint16_t>(J, ref.local, tile, blk, costs);
+                cdef_item<uint16_t>(J, ref.local, tile, blk, taps, costs);
         }
     })
 }

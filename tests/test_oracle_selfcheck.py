@@ -1,0 +1,68 @@
+"""Independent checks of the oracle that the reference's own unit tests leave
+open (CPU only).
+
+Deblocking: the reference walks edges sequentially and in place
+(loop_filter.rs:271-294).  The HIP kernels filter every edge of a pass at
+once, which is the same computation only when no edge reads samples another
+edge of the same pass writes (true for legal, size-aligned block and
+transform geometry).  The oracle's walk is run both ways on generated frames
+and must agree; on a transform grid that is not size-aligned the two must
+differ, which shows the comparison can fail.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import gen_config
+
+FORMATS = [  # (bit_depth, sx, sy, mono)
+    (8, 1, 1, 0), (10, 1, 1, 0), (12, 0, 0, 0), (8, 1, 0, 0), (10, 1, 1, 1),
+]
+SIZES = [(96, 80), (100, 76), (200, 130), (256, 128)]
+
+
+def _frame(seed, **kw):
+    from oxav1 import Frame
+    return Frame.generate(gen_config(**kw), seed)
+
+
+def _lf_both(oracle_mod, f):
+    from oxav1.frames import planes_struct
+    lib = oracle_mod.load()
+    ctx, g, keep = f.lf_args()
+    seq = [p.copy() for p in f.planes]
+    par = [p.copy() for p in f.planes]
+    lib.oo_loop_filter_frame(C.byref(ctx), C.byref(g), planes_struct(seq), len(seq))
+    lib.oo_loop_filter_frame_parallel(C.byref(ctx), C.byref(g), planes_struct(par), len(par))
+    return seq, par
+
+
+@pytest.mark.parametrize("fi", range(len(FORMATS)))
+@pytest.mark.parametrize("si", range(len(SIZES)))
+def test_deblock_sequential_equals_parallel(fi, si, oracle_mod):
+    bd, sx, sy, mono = FORMATS[fi]
+    f = _frame(0x5E0 + 31 * fi + si, upscaled_width=SIZES[si][0], frame_height=SIZES[si][1],
+               bit_depth=bd, subsampling_x=sx, subsampling_y=sy, mono=mono,
+               noise_amp=8 << (bd - 8), exercise_segments=si % 2, exercise_delta_lf=fi % 2,
+               sb128=si % 2)
+    seq, par = _lf_both(oracle_mod, f)
+    assert any(not np.array_equal(a, b) for a, b in zip(seq, f.planes)), "nothing was filtered"
+    for p, (a, b) in enumerate(zip(seq, par)):
+        assert np.array_equal(a, b), f"plane {p}: {(a != b).sum()} samples differ"
+
+
+def test_deblock_parallel_differs_on_overlapping_edges(oracle_mod):
+    """A transform grid that is not size-aligned (16x16 and 4x4 cells mixed at
+    random on 4x4 blocks) puts a 16-wide transform edge next to a 4-wide one,
+    so one edge's taps read samples its neighbour wrote: the in-place walk and
+    the parallel walk must then diverge."""
+    rng = np.random.default_rng(5)
+    f = _frame(0x5E7, upscaled_width=64, frame_height=64, bit_depth=8, noise_amp=2)
+    f.mi_size[:] = 0          # BLOCK_4X4: a block edge every 4 samples
+    f.skip[:] = 0
+    f.lf_tx_size[0][:] = rng.choice([0, 2], size=f.lf_tx_size[0].shape)  # TX_4X4 / TX_16X16
+    f.planes[0][:] = 128 + rng.integers(0, 3, size=f.planes[0].shape)    # flat enough to filter
+    seq, par = _lf_both(oracle_mod, f)
+    assert not np.array_equal(seq[0], f.planes[0])
+    assert not np.array_equal(seq[0], par[0])
