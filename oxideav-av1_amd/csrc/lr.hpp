@@ -25,25 +25,24 @@ constexpr int kLrMaxW = 64;                 // max tile width (plane samples)
 constexpr int kLrMaxH = 64;                 // max stripe height
 constexpr int kLrSw = kLrMaxW + 6;          // source window width
 constexpr int kLrSh = kLrMaxH + 6;          // source window height
-constexpr int kLrAw = kLrMaxW + 2;          // A/B width
-constexpr int kLrAh = kLrMaxH + 6;          // A/B height (pass 1); Wiener rows
-constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // odd rows only (pass 0)
+constexpr int kLrAw = kLrMaxW + 2;          // A/B columns: tile cols -1 .. W
+constexpr int kLrAh1 = kLrMaxH + 2;         // pass-1 A/B rows: tile rows -1 .. H
+constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // pass-0 A/B rows: the odd rows only
 
 __constant__ int32_t c_sgr_params[16][4];   // filled from kSgrParams at init
 __constant__ uint16_t c_sgr_a2[256];        // a2 for 0 < z < 255 (loop_restoration.rs:648-654)
 
+// A (<= 256, 9 bits) and B (< 2^20) of one self-guided position in one word.
+__device__ __forceinline__ uint32_t lr_pack_ab(uint32_t a, uint32_t b) { return (a << 20) | b; }
+
 template <typename L>
 struct LrSmem {
-    L src[kLrSh * kLrSw];
-    union {
-        int32_t inter[kLrAh * kLrAw];  // Wiener horizontal pass, (H+6) rows
-        struct {
-            uint16_t a1[(kLrMaxH + 2) * kLrAw];
-            int32_t b1[(kLrMaxH + 2) * kLrAw];
-            uint16_t a0[kLrAh0 * kLrAw];
-            int32_t b0[kLrAh0 * kLrAw];
-        } sg;
-    } u;
+    L src[kLrSh * kLrSw];              // routed source window, rows ys-3 .. ye+3
+    uint32_t ab1[kLrAh1 * kLrAw];      // pass-1 (A, B)
+    uint32_t ab0[kLrAh0 * kLrAw];      // pass-0 (A, B), odd rows
+    uint64_t rowp[kLrSh];              // routed base address of each window row
+    int32_t rowmax[kLrSh];             // last column of that row's buffer; -1: none
+    uint16_t a2[256];                  // c_sgr_a2 staged for lane-indexed reads
 };
 
 struct LrTileGeom {
@@ -55,56 +54,29 @@ struct LrTileGeom {
     int32_t ww, wh;      // window size
 };
 
-// get_source_sample (loop_restoration.rs:862-902) for a virtual (x, y).
-template <typename T>
-__device__ __forceinline__ int32_t lr_fetch(const LrTileGeom &g, const G<T> *__restrict__ cur, uint32_t cpitch,
-                                            int32_t crows, int32_t ccols, const G<T> *__restrict__ cdf,
-                                            uint32_t dpitch, int32_t drows, int32_t dcols, int32_t x, int32_t y) {
-    x = clip3(0, max(g.pex, 0), x);
-    y = clip3(0, max(g.pey, 0), y);
-    const G<T> *b;
-    uint32_t pitch;
-    int32_t rows, cols;
-    if (y < g.ss) {
-        y = max(y, g.ss - 2);
-        b = cur, pitch = cpitch, rows = crows, cols = ccols;
-    } else if (y > g.se) {
-        y = min(y, g.se + 2);
-        b = cur, pitch = cpitch, rows = crows, cols = ccols;
-    } else {
-        b = cdf, pitch = dpitch, rows = drows, cols = dcols;
-    }
-    if (rows == 0 || cols == 0) return 0;
-    y = clip3(0, rows - 1, y);
-    x = clip3(0, cols - 1, x);
-    return (int32_t)b[(size_t)y * pitch + x];
+// Self-guided A/B from box sums (loop_restoration.rs:615-665): a = sum of
+// squares, b = sum over the (2r+1)^2 window.  Returns lr_pack_ab(A, B).
+__device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, int32_t n, uint32_t s, uint32_t one_over_n,
+                                          uint32_t bd, const uint16_t *a2tab) {
+    const uint32_t sha = 2 * (bd - 8), shb = bd - 8;
+    const uint32_t ar = sha ? ((a + (1u << (sha - 1))) >> sha) : a;
+    const int32_t dr = shb ? ((b + (1 << (shb - 1))) >> shb) : b;
+    int32_t p = (int32_t)ar * n - dr * dr;
+    p = p < 0 ? 0 : p;
+    // p * s < 2^32 at every depth (p < 41.2e6 with s <= 76 for r = 2, p < 5.3e6
+    // with s <= 15 for r = 1)
+    const uint32_t z = ((uint32_t)p * s + (1u << 19)) >> 20;
+    const uint32_t a2 = z >= 255 ? 256u : (z == 0 ? 1u : (uint32_t)a2tab[z]);
+    // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth
+    const uint32_t b2 = (256u - a2) * (uint32_t)b * one_over_n;
+    return lr_pack_ab(a2, (b2 + 2048u) >> 12);
 }
 
-// Self-guided A/B at one position (loop_restoration.rs:615-665) from the
-// LDS window; (wy, wx) is the window coordinate of the centre.
-template <typename L>
-__device__ __forceinline__ void lr_ab(const L *src, int32_t wy, int32_t wx, int32_t r, int32_t n, uint32_t s,
-                                      int32_t one_over_n, uint32_t bd, uint16_t *A, int32_t *B) {
-    int32_t a = 0, b = 0;
-    for (int32_t dy = -r; dy <= r; dy++) {
-        const L *row = src + (wy + dy) * kLrSw + wx;
-        for (int32_t dx = -r; dx <= r; dx++) {
-            const int32_t c = (int32_t)row[dx];
-            a += c * c;
-            b += c;
-        }
-    }
-    const uint32_t sha = 2 * (bd - 8), shb = bd - 8;
-    const int32_t ar = sha ? (int32_t)(((uint32_t)a + (1u << (sha - 1))) >> sha) : a;
-    const int32_t dr = shb ? ((b + (1 << (shb - 1))) >> shb) : b;
-    int32_t p = ar * n - dr * dr;
-    p = p < 0 ? 0 : p;
-    const uint32_t z = (uint32_t)(((uint64_t)(uint32_t)p * s + (1u << 19)) >> 20);
-    const int32_t a2 = z >= 255 ? 256 : (z == 0 ? 1 : (int32_t)c_sgr_a2[z]);
-    // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth
-    const uint32_t b2 = (uint32_t)(256 - a2) * (uint32_t)b * (uint32_t)one_over_n;
-    *A = (uint16_t)a2;
-    *B = (int32_t)((b2 + 2048u) >> 12);
+__device__ __forceinline__ void lr_sgr_consts(int32_t r, int32_t e, int32_t *n, uint32_t *s, uint32_t *oon) {
+    *n = (2 * r + 1) * (2 * r + 1);
+    const int64_t n2e = (int64_t)(*n) * (*n) * e;
+    *s = (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
+    *oon = (uint32_t)(((1 << 12) + *n / 2) / *n);
 }
 
 // One tile: (plane, stripe, column tile) -> out (cropped frame coordinates
@@ -124,18 +96,15 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
     const int32_t tw = (int32_t)J.lr_tile_w[plane];
     g.xs = (int32_t)tile * tw;
     g.xe = min(g.xs + tw - 1, g.pex);
-    if (g.ys > g.ye || g.xs > g.xe) return;
-    const G<T> *cur = plane_ptr<T>(J.up_deb[plane]);
-    const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
-    const uint32_t cpitch = J.up_deb_pitch[plane], dpitch = J.up_cdef_pitch[plane];
-    const int32_t rows = (int32_t)J.lr_rows[plane], cols = (int32_t)J.lr_cols[plane];
-    const int32_t crows = (int32_t)J.lrc_rows[plane], ccols = (int32_t)J.lrc_cols[plane];
-    const int tid = threadIdx.x, nt = blockDim.x;
     // writes are dropped outside the output buffer (loop_restoration.rs:814-818)
     g.ye = min(g.ye, (int32_t)J.out_rows[plane] - 1);
     g.xe = min(g.xe, (int32_t)J.out_cols[plane] - 1);
     if (g.ys > g.ye || g.xs > g.xe) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wv = tid >> 6, nwv = nt >> 6;
     const int32_t H = g.ye - g.ys + 1, W = g.xe - g.xs + 1;
+    const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
+    const uint32_t dpitch = J.up_cdef_pitch[plane];
 
     // unit of this tile (loop_restoration.rs:395-405)
     uint32_t type = J.uses_lr ? J.lr_type[plane] : OXAV1_RESTORE_NONE;
@@ -147,147 +116,238 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
         u = J.lr_units[plane] ? J.lr_units[plane][(size_t)ur * J.lr_unit_cols[plane] + uc] : oxav1_lr_unit{};
         type = u.type;
     }
+    // Output rows are split into one band per wave; a lane owns a column.
+    const int RB = (H + nwv - 1) / nwv;
+    const int b0 = wv * RB, b1 = min(H, b0 + RB);
     if (type != OXAV1_RESTORE_WIENER && type != OXAV1_RESTORE_SGRPROJ) {
         if (!J.lr_copy_none) return;
-        for (int i = tid; i < H * W; i += nt) {
-            const int yy = i / W, xx = i - yy * W;
-            const int32_t y = g.ys + yy, x = g.xs + xx;
-            out[(size_t)y * opitch + x] = cdf[(size_t)y * dpitch + x];
-        }
+        for (int x = lane; x < W; x += 64)
+            for (int r = b0; r < b1; r++)
+                out[(size_t)(g.ys + r) * opitch + (g.xs + x)] = cdf[(size_t)(g.ys + r) * dpitch + (g.xs + x)];
         return;
     }
-    // routed source window
+    // Routed source window (get_source_sample, loop_restoration.rs:862-902):
+    // rows above/below the stripe come from the pre-CDEF frame (at most 2
+    // rows out), the rest from the CDEF frame; then both clamps.
     g.wy0 = g.ys - 3;
     g.wx0 = g.xs - 3;
     g.wh = H + 6;
     g.ww = W + 6;
-    for (int i = tid; i < g.wh * g.ww; i += nt) {
-        const int yy = i / g.ww, xx = i - yy * g.ww;
-        S.src[yy * kLrSw + xx] =
-            (L)lr_fetch<T>(g, cur, cpitch, crows, ccols, cdf, dpitch, rows, cols, g.wx0 + xx, g.wy0 + yy);
+    if (tid < g.wh) {
+        const int32_t crows = (int32_t)J.lrc_rows[plane], ccols = (int32_t)J.lrc_cols[plane];
+        const int32_t rows = (int32_t)J.lr_rows[plane], cols = (int32_t)J.lr_cols[plane];
+        int32_t y = clip3(0, max(g.pey, 0), g.wy0 + tid);
+        const G<T> *b = cdf;
+        uint32_t pitch = dpitch;
+        int32_t nr = rows, nc = cols;
+        if (y < g.ss || y > g.se) {
+            y = y < g.ss ? max(y, g.ss - 2) : min(y, g.se + 2);
+            b = plane_ptr<T>(J.up_deb[plane]);
+            pitch = J.up_deb_pitch[plane];
+            nr = crows;
+            nc = ccols;
+        }
+        if (nr == 0 || nc == 0) {
+            S.rowp[tid] = 0;
+            S.rowmax[tid] = -1;
+        } else {
+            y = clip3(0, nr - 1, y);
+            S.rowp[tid] = (uint64_t)(b + (size_t)y * pitch);
+            S.rowmax[tid] = nc - 1;
+        }
+    }
+    if (type == OXAV1_RESTORE_SGRPROJ)
+        for (int i = tid; i < 256; i += nt) S.a2[i] = c_sgr_a2[i];
+    __syncthreads();
+    {
+        const uint32_t m = ((1u << 20) + (uint32_t)g.ww - 1) / (uint32_t)g.ww;
+        const int32_t xmax = max(g.pex, 0);
+        const int n = g.wh * g.ww;
+#pragma unroll 4
+        for (int i = tid; i < n; i += nt) {
+            const int yy = (int)(((uint32_t)i * m) >> 20), xx = i - yy * g.ww;  // exact: i < 2^20 / 70
+            const int32_t cm = S.rowmax[yy];
+            const int32_t x = clip3(0, cm, clip3(0, xmax, g.wx0 + xx));
+            const G<T> *row = (const G<T> *)S.rowp[yy];
+            S.src[yy * kLrSw + xx] = cm < 0 ? (L)0 : (L)row[x];
+        }
     }
     __syncthreads();
     const int32_t bd = J.bd;
     const int32_t maxv = (1 << bd) - 1;
     if (type == OXAV1_RESTORE_WIENER) {
-        // wiener_filter (loop_restoration.rs:742-821)
-        int32_t vf[7], hf[7];
+        // wiener_filter (loop_restoration.rs:742-821); both 7-tap filters are
+        // symmetric, so taps t and 6-t share one multiply
+        int32_t vf[4], hf[4];
         vf[3] = hf[3] = 128;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            vf[i] = vf[6 - i] = u.wiener[0][i];
-            hf[i] = hf[6 - i] = u.wiener[1][i];
+            vf[i] = u.wiener[0][i];
+            hf[i] = u.wiener[1][i];
             vf[3] -= 2 * u.wiener[0][i];
             hf[3] -= 2 * u.wiener[1][i];
         }
         const uint32_t r0 = bd == 12 ? 5 : 3, r1 = bd == 12 ? 9 : 11;
         const int32_t offset = 1 << (bd + 7 - (int32_t)r0 - 1);
         const int32_t limit = (1 << (bd + 1 + 7 - (int32_t)r0)) - 1;
-        int32_t *inter = S.u.inter;  // (H+6) x W, stride kLrAw
-        for (int i = tid; i < (H + 6) * W; i += nt) {
-            const int rr = i / W, c = i - rr * W;
-            const L *s = S.src + rr * kLrSw + c;  // window col c == virtual xs + c - 3
-            int32_t acc = 0;
-#pragma unroll
-            for (int t = 0; t < 7; t++) acc += hf[t] * (int32_t)s[t];
-            inter[rr * kLrAw + c] = clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
-        }
-        __syncthreads();
-        for (int i = tid; i < H * W; i += nt) {
-            const int rr = i / W, c = i - rr * W;
-            int32_t acc = 0;
-#pragma unroll
-            for (int t = 0; t < 7; t++) acc += vf[t] * inter[(rr + t) * kLrAw + c];
-            const int32_t v = (acc + (1 << (r1 - 1))) >> r1;
-            out[(size_t)(g.ys + rr) * opitch + (g.xs + c)] = (T)clip3(0, maxv, v);
+        if (lane >= W || b0 >= b1) return;
+        // horizontal pass of window row wr at this lane's column
+        auto hrow = [&](int wr) -> int32_t {
+            const L *q = S.src + wr * kLrSw + lane;
+            const int32_t acc = hf[0] * ((int32_t)q[0] + (int32_t)q[6]) + hf[1] * ((int32_t)q[1] + (int32_t)q[5]) +
+                                hf[2] * ((int32_t)q[2] + (int32_t)q[4]) + hf[3] * (int32_t)q[3];
+            return clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
+        };
+        int32_t h0 = hrow(b0), h1 = hrow(b0 + 1), h2 = hrow(b0 + 2), h3 = hrow(b0 + 3), h4 = hrow(b0 + 4),
+                h5 = hrow(b0 + 5);
+        G<T> *o = out + (size_t)(g.ys + b0) * opitch + (g.xs + lane);
+        for (int r = b0; r < b1; r++, o += opitch) {
+            const int32_t h6 = hrow(r + 6);
+            const int32_t acc = vf[0] * (h0 + h6) + vf[1] * (h1 + h5) + vf[2] * (h2 + h4) + vf[3] * h3;
+            *o = (T)clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1);
+            h0 = h1, h1 = h2, h2 = h3, h3 = h4, h4 = h5, h5 = h6;
         }
         return;
     }
-    // self_guided_filter (loop_restoration.rs:485-579) + box_filter
+    // self_guided_filter (loop_restoration.rs:485-579) + box_filter (594-729)
     const uint32_t set = u.sgr_set < 15 ? u.sgr_set : 15;
     const int32_t r0 = c_sgr_params[set][0], e0 = c_sgr_params[set][1];
     const int32_t r1 = c_sgr_params[set][2], e1 = c_sgr_params[set][3];
     const int32_t w0 = u.sgr_xqd[0], w1 = u.sgr_xqd[1];
     const int32_t w2 = 128 - w0 - w1;
-    auto params = [](int32_t r, int32_t e, int32_t *n, uint32_t *s, int32_t *oon) {
-        *n = (2 * r + 1) * (2 * r + 1);
-        const int64_t n2e = (int64_t)(*n) * (*n) * e;
-        *s = (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
-        *oon = ((1 << 12) + *n / 2) / *n;
-    };
-    // A/B rows: virtual ys-1 .. ye+1 -> index 0 .. H+1; window row = index + 2
-    const int AW = W + 2;
-    if (r0 != 0) {
-        int32_t n;
-        uint32_t s;
-        int32_t oon;
-        params(r0, e0, &n, &s, &oon);
-        // odd absolute rows only: virtual row vy odd; index k = (vy - (ys-1)) >> 1
-        // ys is even, so ys-1 is odd and vy = ys-1+2k.
-        const int AH0 = (H + 2 + 1) / 2;
-        for (int i = tid; i < AH0 * AW; i += nt) {
-            const int k = i / AW, c = i - k * AW;
-            const int wy = 2 * k + 2, wx = c + 2;
-            lr_ab<L>(S.src, wy, wx, r0, n, s, oon, (uint32_t)bd, &S.u.sg.a0[k * kLrAw + c], &S.u.sg.b0[k * kLrAw + c]);
-        }
-    }
-    if (r1 != 0) {
-        int32_t n;
-        uint32_t s;
-        int32_t oon;
-        params(r1, e1, &n, &s, &oon);
-        for (int i = tid; i < (H + 2) * AW; i += nt) {
-            const int rr = i / AW, c = i - rr * AW;
-            lr_ab<L>(S.src, rr + 2, c + 2, r1, n, s, oon, (uint32_t)bd, &S.u.sg.a1[rr * kLrAw + c], &S.u.sg.b1[rr * kLrAw + c]);
+    // A/B over tile rows -1 .. H and cols -1 .. W: thread -> (column, row
+    // band); each walks down its band keeping 3- and 5-wide horizontal sums
+    // of the last five window rows, so every box sum costs five LDS reads.
+    // Pass 0 (radius r0 = 2 when used) is needed on odd rows only: its
+    // even-row output weights are 0 and block rows start even
+    // (loop_restoration.rs:686-699).
+    {
+        const int ncol = W + 2;
+        const int nb = nt / ncol;
+        const int c = tid % ncol, band = tid / ncol;
+        const int KR = (H + 2 + nb - 1) / nb;
+        const int k0 = -1 + band * KR, k1 = min(H, k0 + KR - 1);  // inclusive tile rows
+        if (band < nb && k0 <= k1) {
+            int32_t n0 = 0, n1 = 0;
+            uint32_t s0 = 0, s1 = 0, o0 = 0, o1 = 0;
+            if (r0) lr_sgr_consts(r0, e0, &n0, &s0, &o0);
+            if (r1) lr_sgr_consts(r1, e1, &n1, &s1, &o1);
+            // window row of tile row k is k + 3; column of tile col c - 1 is c + 2
+            int32_t h3[5], q3[5], h5[5], q5[5];
+            auto hsum = [&](int wr, int slot) {
+                const L *q = S.src + wr * kLrSw + c;  // cols c .. c+4 = tile cols c-3 .. c+1
+                const int32_t v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3], v4 = q[4];
+                h3[slot] = v1 + v2 + v3;
+                q3[slot] = v1 * v1 + v2 * v2 + v3 * v3;
+                h5[slot] = h3[slot] + v0 + v4;
+                q5[slot] = q3[slot] + v0 * v0 + v4 * v4;
+            };
+            // rows k0-2 .. k0+1 preloaded into slots 0..3
+#pragma unroll
+            for (int j = 0; j < 4; j++) hsum(k0 - 2 + j + 3, j);
+            const int32_t ys = g.ys;
+            for (int k = k0; k <= k1; k++) {
+                hsum(k + 2 + 3, 4);
+                if (r1) {
+                    uint32_t a;
+                    int32_t b;
+                    if (r1 == 1) {
+                        a = (uint32_t)(q3[1] + q3[2] + q3[3]);
+                        b = h3[1] + h3[2] + h3[3];
+                    } else {
+                        a = (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
+                        b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
+                    }
+                    S.ab1[(k + 1) * kLrAw + c] = lr_ab(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
+                }
+                if (r0 && ((ys + k) & 1)) {
+                    const uint32_t a =
+                        (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
+                    const int32_t b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
+                    S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    h3[j] = h3[j + 1];
+                    q3[j] = q3[j + 1];
+                    h5[j] = h5[j + 1];
+                    q5[j] = q5[j + 1];
+                }
+            }
         }
     }
     __syncthreads();
-    for (int i = tid; i < H * W; i += nt) {
-        const int rr = i / W, c = i - rr * W;
-        const int32_t uval = (int32_t)S.src[(rr + 3) * kLrSw + (c + 3)];
+    if (lane >= W || b0 >= b1) return;
+    // Weighted 3x3 of A/B (loop_restoration.rs:686-729), walking down the
+    // band: pass 1 weights 3 on corners and 4 elsewhere, i.e. for rows
+    // k-1, k+1: 3*S + M, row k: 4*S (S = three-column sum, M = centre);
+    // pass 0 (odd rows R hold P(R) = 5*(left + right) + 6*centre): odd rows
+    // use P(k) >> 4, even rows (P(k-1) + P(k+1)) >> 5.
+    const int32_t ys = g.ys;
+    auto row3 = [&](const uint32_t *ab, int idx, int32_t &sa, int32_t &ma, int32_t &sb, int32_t &mb) {
+        const uint32_t l = ab[idx + lane], m = ab[idx + lane + 1], r = ab[idx + lane + 2];
+        ma = (int32_t)(m >> 20);
+        mb = (int32_t)(m & 0xFFFFFu);
+        sa = (int32_t)(l >> 20) + ma + (int32_t)(r >> 20);
+        sb = (int32_t)(l & 0xFFFFFu) + mb + (int32_t)(r & 0xFFFFFu);
+    };
+    auto prow = [&](int k, int32_t &pa, int32_t &pb) {  // P at odd tile row k
+        const int idx = ((k + 1) >> 1) * kLrAw;
+        const uint32_t l = S.ab0[idx + lane], m = S.ab0[idx + lane + 1], r = S.ab0[idx + lane + 2];
+        pa = 5 * ((int32_t)(l >> 20) + (int32_t)(r >> 20)) + 6 * (int32_t)(m >> 20);
+        pb = 5 * ((int32_t)(l & 0xFFFFFu) + (int32_t)(r & 0xFFFFFu)) + 6 * (int32_t)(m & 0xFFFFFu);
+    };
+    int32_t sa0 = 0, ma0 = 0, sb0 = 0, mb0 = 0, sa1 = 0, ma1 = 0, sb1 = 0, mb1 = 0;
+    if (r1) {
+        row3(S.ab1, (b0 - 1 + 1) * kLrAw, sa0, ma0, sb0, mb0);
+        row3(S.ab1, (b0 + 1) * kLrAw, sa1, ma1, sb1, mb1);
+    }
+    // pass-0 P of the odd row at or above the current row (tile row -1 is odd)
+    int32_t pa_prev = 0, pb_prev = 0;
+    if (r0) {
+        const int kp = ((ys + b0) & 1) ? b0 : b0 - 1;
+        prow(kp, pa_prev, pb_prev);
+    }
+    G<T> *o = out + (size_t)(ys + b0) * opitch + (g.xs + lane);
+    for (int k = b0; k < b1; k++, o += opitch) {
+        const int32_t uval = (int32_t)S.src[(k + 3) * kLrSw + lane + 3];
         const int32_t u4 = uval << 4;
         int32_t f0 = u4, f1 = u4;
-        if (r0 != 0) {
-            // row parity relative to the block == absolute parity (block rows are even)
-            const int32_t vy = g.ys + rr;
+        if (r0) {
             int32_t as, bs;
             uint32_t sh;
-            if (vy & 1) {
-                const int k = (vy - (g.ys - 1)) >> 1;  // this (odd) row
-                const uint16_t *a = S.u.sg.a0 + k * kLrAw + c;
-                const int32_t *b = S.u.sg.b0 + k * kLrAw + c;
-                as = 5 * a[0] + 6 * a[1] + 5 * a[2];
-                bs = 5 * b[0] + 6 * b[1] + 5 * b[2];
+            if ((ys + k) & 1) {  // P(k) is already held: set at b0 or by row k-1
+                as = pa_prev;
+                bs = pb_prev;
                 sh = 4;
             } else {
-                const int k = (vy - 1 - (g.ys - 1)) >> 1;  // rows vy-1 and vy+1
-                const uint16_t *a = S.u.sg.a0 + k * kLrAw + c;
-                const int32_t *b = S.u.sg.b0 + k * kLrAw + c;
-                const uint16_t *a2 = a + kLrAw;
-                const int32_t *b2 = b + kLrAw;
-                as = 5 * (a[0] + a[2] + a2[0] + a2[2]) + 6 * (a[1] + a2[1]);
-                bs = 5 * (b[0] + b[2] + b2[0] + b2[2]) + 6 * (b[1] + b2[1]);
+                int32_t pa_next, pb_next;
+                prow(k + 1, pa_next, pb_next);
+                as = pa_prev + pa_next;
+                bs = pb_prev + pb_next;
+                pa_prev = pa_next;
+                pb_prev = pb_next;
                 sh = 5;
             }
             const int32_t v = as * uval + bs;
             const uint32_t ts = 8 + sh - 4;
             f0 = (v + (1 << (ts - 1))) >> ts;
         }
-        if (r1 != 0) {
-            const uint16_t *a = S.u.sg.a1 + rr * kLrAw + c;
-            const int32_t *b = S.u.sg.b1 + rr * kLrAw + c;
-            const int32_t as = 3 * (a[0] + a[2] + a[2 * kLrAw] + a[2 * kLrAw + 2]) +
-                               4 * (a[1] + a[kLrAw] + a[kLrAw + 1] + a[kLrAw + 2] + a[2 * kLrAw + 1]);
-            const int32_t bs = 3 * (b[0] + b[2] + b[2 * kLrAw] + b[2 * kLrAw + 2]) +
-                               4 * (b[1] + b[kLrAw] + b[kLrAw + 1] + b[kLrAw + 2] + b[2 * kLrAw + 1]);
+        if (r1) {
+            int32_t sa2, ma2, sb2, mb2;
+            row3(S.ab1, (k + 2) * kLrAw, sa2, ma2, sb2, mb2);
+            const int32_t as = 3 * (sa0 + sa2) + ma0 + ma2 + 4 * sa1;
+            const int32_t bs = 3 * (sb0 + sb2) + mb0 + mb2 + 4 * sb1;
             const int32_t v = as * uval + bs;
             f1 = (v + (1 << 8)) >> 9;
+            sa0 = sa1, ma0 = ma1, sb0 = sb1, mb0 = mb1;
+            sa1 = sa2, ma1 = ma2, sb1 = sb2, mb1 = mb2;
         }
         int32_t v = w1 * u4;
         v += r0 != 0 ? w0 * f0 : w0 * u4;
         v += r1 != 0 ? w2 * f1 : w2 * u4;
         const int32_t sres = (v + (1 << 10)) >> 11;
-        out[(size_t)(g.ys + rr) * opitch + (g.xs + c)] = (T)clip3(0, maxv, sres);
+        *o = (T)clip3(0, maxv, sres);
     }
 }
 
