@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=8, help="frames per step (batch)")
     ap.add_argument("--cfg", type=int, default=2, help="workload configuration (2 = 4K 10-bit)")
-    ap.add_argument("--cpu-frames", type=int, default=2, help="frames timed on the CPU oracle")
+    ap.add_argument("--cpu-frames", type=int, default=4, help="frames timed on the CPU oracle (about 10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="timed batched steps only (for rocprofv3)")
     ap.add_argument("--copy-bytes", type=int, default=0,

@@ -56,6 +56,8 @@ _SIGS = {
     "oo_loop_filter_frame": (None, [_P(abi.LfCtx), _P(abi.LfGrids), _P(abi.Plane), C.c_uint32]),
     "oo_loop_filter_frame_parallel": (None, [_P(abi.LfCtx), _P(abi.LfGrids), _P(abi.Plane),
                                              C.c_uint32]),
+    "oo_loop_filter_frame_reversed": (None, [_P(abi.LfCtx), _P(abi.LfGrids), _P(abi.Plane),
+                                             C.c_uint32]),
     "oo_cdef_frame": (None, [_P(abi.CdefCtx), _P(abi.CdefGrids), _P(abi.Plane), C.c_uint32,
                              _P(abi.Plane), C.c_uint32]),
     "oo_upscale_frame": (C.c_int, [_P(abi.SrCtx), _P(abi.Plane), C.c_uint32, _P(abi.Plane),

@@ -410,6 +410,26 @@ void oo_loop_filter_frame_parallel(const oxav1_lf_ctx *ctx, const oxav1_lf_grids
     }
 }
 
+/* The walk of oo_loop_filter_frame with the mi rows and columns of each
+ * (plane, pass) visited in reverse order: agrees with the forward walk
+ * exactly when no edge's filtering depends on the order of the walk. */
+void oo_loop_filter_frame_reversed(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                                   oxav1_plane_i32 *planes, uint32_t n) {
+    uint32_t np = ctx->num_planes < n ? ctx->num_planes : n;
+    for (uint32_t plane = 0; plane < np; plane++) {
+        if (plane > 0 && ctx->loop_filter_level[1 + plane] == 0) continue;
+        uint32_t sx = plane ? ctx->subsampling_x : 0, sy = plane ? ctx->subsampling_y : 0;
+        uint32_t row_step = plane == 0 ? 1 : (1u << sy);
+        uint32_t col_step = plane == 0 ? 1 : (1u << sx);
+        uint32_t nr = (ctx->mi_rows + row_step - 1) / row_step, nc = (ctx->mi_cols + col_step - 1) / col_step;
+        for (uint8_t pass = 0; pass < 2; pass++)
+            for (uint32_t ri = nr; ri-- > 0;)
+                for (uint32_t ci = nc; ci-- > 0;)
+                    oo_loop_filter_edge(ctx, g, planes, n, (uint8_t)plane, pass, ri * row_step,
+                                        ci * col_step);
+    }
+}
+
 /* ===================================================================== */
 /* §7.15 CDEF                                                             */
 /* ===================================================================== */

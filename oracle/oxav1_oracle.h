@@ -38,6 +38,9 @@ void oo_loop_filter_frame(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
 /* Test helper: every edge of a pass reads the pass-start samples. */
 void oo_loop_filter_frame_parallel(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
                                    oxav1_plane_i32 *planes, uint32_t n);
+/* Test helper: the same walk with rows and columns visited in reverse. */
+void oo_loop_filter_frame_reversed(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
+                                   oxav1_plane_i32 *planes, uint32_t n);
 void oo_loop_filter_edge(const oxav1_lf_ctx *ctx, const oxav1_lf_grids *g,
                          oxav1_plane_i32 *planes, uint32_t n, uint8_t plane, uint8_t pass,
                          uint32_t row, uint32_t col);

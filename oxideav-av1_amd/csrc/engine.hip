@@ -126,7 +126,7 @@ __device__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
     __syncthreads();
     const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
     const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
-    fg_blend_rows<T>(J, p, y0, y1, x0, x1, lut);
+    fg_blend_tile<T>(J, p, y0, y1, x0, x1, lut);
 }
 
 // --- kernels: KIND 0 = engine (u8/u16 by FrameJob::kind), 1 = int32 ---
@@ -155,7 +155,7 @@ template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_lf(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
-    __shared__ L region[kLfRegion * kLfRegion];
+    __shared__ L region[kLfRegion * kLfStride];
     __shared__ LfShared sh;
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {

@@ -325,9 +325,15 @@ __device__ __forceinline__ int32_t fg_scale(const uint8_t *t, int32_t index, uin
     return a + fg_round2((b - a) * rem, shift);
 }
 
-// Blend rows [y0, y1) of one plane: in (pre-grain) -> out.
+// Blend one tile (rows [y0, y1), cols [x0, x1), at most 64 x 64, tile
+// corners on multiples of 64) of one plane: in (pre-grain) -> out
+// (add_noise_synthesis, film_grain.rs:570-829).  A lane owns a column and
+// each wave a band of 16 rows.  Bands never straddle a noise stripe (32 or
+// 16 rows) and lanes never straddle a noise block, so the block offsets and
+// template row pointers are resolved once per lane, and the overlap blends
+// only touch the first columns of a block and the first rows of a stripe.
 template <typename T>
-__device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
+__device__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
                               const uint8_t *lut /* LDS [3][256] */) {
     const G<FgScratch> *S = (const G<FgScratch> *)J.fgs;
     const oxav1_film_grain_params &fg = J.fg;
@@ -350,7 +356,10 @@ __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int
     G<T> *out = plane_ptr<T>(J.out[plane]);
     const uint32_t ip = J.ref_pitch[plane], op = J.out_pitch[plane];
     const int32_t w = (int32_t)J.out_cols[0];
-    const int32_t cols = x1 - x0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t x = x0 + lane;
+    const int32_t yb0 = y0 + 16 * wv, yb1 = min(y1, yb0 + 16);
+    if (x >= x1 || yb0 >= yb1) return;
     bool active;
     const G<int16_t> *tmpl;
     int tw;
@@ -366,37 +375,80 @@ __device__ void fg_blend_rows(const FrameJob &J, uint32_t plane, int32_t y0, int
         tmpl = plane == 1 ? S->cb : S->cr;
         tw = S->chroma_w;
     }
+    const G<T> *src = in + (size_t)yb0 * ip + x;
+    G<T> *dst = out + (size_t)yb0 * op + x;
+    if (!active) {
+        for (int32_t y = yb0; y < yb1; y++, src += ip, dst += op) *dst = *src;
+        return;
+    }
+    // noise block / stripe geometry (film_grain.rs:626-651, 705-739)
+    const uint32_t bsh = 5 - psx, ssh = 5 - psy;       // log2 block width / stripe height
+    const uint32_t b = (uint32_t)x >> bsh;
+    const int32_t j = x & ((1 << bsh) - 1);
+    const uint32_t st = (uint32_t)yb0 >> ssh;
+    const int32_t ib = yb0 - (int32_t)(st << ssh);       // row of yb0 inside its stripe
+    const G<uint8_t> *offs = (const G<uint8_t> *)S->offsets;
+    const uint32_t nblocks = S->nblocks;
+    auto block_base = [&](uint32_t ss, uint32_t bb) -> const G<int16_t> * {
+        const uint32_t rnd = offs[ss * nblocks + bb];
+        const int32_t ox = (int32_t)(rnd >> 4), oy = (int32_t)(rnd & 15u);
+        const int32_t pox = psx ? 6 + ox : 9 + 2 * ox;
+        const int32_t poy = psy ? 6 + oy : 9 + 2 * oy;
+        return tmpl + poy * tw + pox;
+    };
+    // horizontal overlap: the first 2 (1 when subsampled) columns of a block
+    // blend with the previous block's columns 32, 33 (16)
+    const bool hov = overlap && b > 0 && j < (psx ? 1 : 2);
+    const int32_t hw_old = psx ? 23 : (j == 0 ? 27 : 17), hw_new = psx ? 22 : (j == 0 ? 17 : 27);
+    const G<int16_t> *pc = block_base(st, b) + j;
+    const G<int16_t> *pl = hov ? block_base(st, b - 1) + j + (1 << bsh) : pc;
+    auto hval = [&](const G<int16_t> *c, const G<int16_t> *l, int32_t i) -> int32_t {
+        int32_t g = c[i * tw];
+        if (hov) g = clip3(gmin, gmax, fg_round2(l[i * tw] * hw_old + g * hw_new, 5));
+        return g;
+    };
+    // vertical overlap: the first 2 (1) rows of a stripe blend with rows 32,
+    // 33 (16) of the previous stripe's noise
+    const int32_t vrows = (overlap && st > 0) ? (psy ? 1 : 2) : 0;
+    const G<int16_t> *pu = pc, *pul = pl;
+    if (vrows > ib) {
+        pu = block_base(st - 1, b) + j;
+        pul = hov ? block_base(st - 1, b - 1) + j + (1 << bsh) : pu;
+    }
+    const uint8_t *lt = lut + 256 * plane;
     const G<T> *ly = plane_ptr<T>(J.ref_out[0]);
     const uint32_t lp = J.ref_pitch[0];
-    for (int32_t i = threadIdx.x; i < (y1 - y0) * cols; i += blockDim.x) {
-        const int32_t y = y0 + i / cols, x = x0 + i % cols;
-        const int32_t orig = (int32_t)in[(size_t)y * ip + x];
-        if (!active) {
-            out[(size_t)y * op + x] = (T)orig;
-            continue;
+    const int32_t lx = x << psx, lnx = min(lx + 1, w - 1);
+    const bool from_luma = fg.chroma_scaling_from_luma != 0;
+    const int32_t lm = plane == 1 ? fg.cb_luma_mult : fg.cr_luma_mult;
+    const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
+    const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) << (bd - 8);
+    const int32_t maxs = (1 << bd) - 1;
+    for (int32_t y = yb0, i = ib; y < yb1; y++, i++, src += ip, dst += op) {
+        int32_t nz = hval(pc, pl, i);
+        if (i < vrows) {
+            const int32_t old = hval(pu, pul, i + (1 << ssh));
+            const int32_t wo = psy ? 23 : (i == 0 ? 27 : 17), wn = psy ? 22 : (i == 0 ? 17 : 27);
+            nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
         }
-        const int32_t nz = fg_noise(tmpl, tw, (const G<uint8_t> *)S->offsets, S->nblocks, y, x, psx, psy, overlap, gmin, gmax);
+        const int32_t orig = (int32_t)*src;
         if (plane == 0) {
-            const int32_t n2 = fg_round2(fg_scale(lut, orig, bd) * nz, gs);
-            out[(size_t)y * op + x] = (T)clip3(minv, maxl, orig + n2);
+            const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
+            *dst = (T)clip3(minv, maxl, orig + n2);
             continue;
         }
-        const int32_t lx = x << psx, lyy = y << psy;
-        const int32_t lnx = min(lx + 1, w - 1);
-        const int32_t avg = psx ? fg_round2((int32_t)ly[(size_t)lyy * lp + lx] + (int32_t)ly[(size_t)lyy * lp + lnx], 1)
-                                : (int32_t)ly[(size_t)lyy * lp + lx];
+        // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
+        const G<T> *lrow = ly + (size_t)(y << psy) * lp;
+        const int32_t avg = psx ? fg_round2((int32_t)lrow[lx] + (int32_t)lrow[lnx], 1) : (int32_t)lrow[lx];
         int32_t merged;
-        if (fg.chroma_scaling_from_luma) {
+        if (from_luma) {
             merged = avg;
         } else {
-            const int32_t lm = plane == 1 ? fg.cb_luma_mult : fg.cr_luma_mult;
-            const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
-            const int32_t off = plane == 1 ? fg.cb_offset : fg.cr_offset;
             const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
-            merged = clip3(0, (1 << bd) - 1, (comb >> 6) + ((off - 256) << (bd - 8)));
+            merged = clip3(0, maxs, (comb >> 6) + coff);
         }
-        const int32_t n2 = fg_round2(fg_scale(lut + 256 * plane, merged, bd) * nz, gs);
-        out[(size_t)y * op + x] = (T)clip3(minv, maxc, orig + n2);
+        const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
+        *dst = (T)clip3(minv, maxc, orig + n2);
     }
 }
 

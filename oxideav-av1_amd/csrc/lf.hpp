@@ -26,6 +26,8 @@ namespace oxav1 {
 constexpr int kLfTile = 64;
 constexpr int kLfHalo = 12;
 constexpr int kLfRegion = kLfTile + 2 * kLfHalo;   // 88
+constexpr int kLfStride = kLfRegion + 2;           // LDS row pitch: 45 dwords of u16, odd, so
+                                                   // lanes walking down a column hit distinct banks
 constexpr int kLfEdges = kLfTile / 4 + 3;          // edges at X0-4 .. X0+TW+4
 constexpr int kLfCellRows = kLfRegion / 4;         // 22
 
@@ -124,29 +126,55 @@ __device__ uint32_t lf_decide(const FrameJob &J, uint32_t plane, uint32_t pass, 
 
 // LDS view of the tile region with the reference's clamped reads / dropped
 // writes.  Region origin (ry0, rx0) may be negative; only in-plane entries are
-// loaded and every clamped coordinate lands inside the region.
-template <typename L>
+// loaded and every clamped coordinate lands inside the region.  EDGE = false
+// when the whole region lies inside the plane: every tap of every edge the
+// tile filters then stays inside the plane and the region, so the clamps and
+// the write test are no-ops and are skipped.
+template <typename L, bool EDGE>
 struct LfRegion {
     L *s;
     int32_t ry0, rx0, rows, cols;
     __device__ __forceinline__ int idx(int32_t y, int32_t x) const {
-        y = clip3(0, rows - 1, y);
-        x = clip3(0, cols - 1, x);
-        return (y - ry0) * kLfRegion + (x - rx0);
+        if (EDGE) {
+            y = clip3(0, rows - 1, y);
+            x = clip3(0, cols - 1, x);
+        }
+        return (y - ry0) * kLfStride + (x - rx0);
     }
     __device__ __forceinline__ int32_t get(int32_t y, int32_t x) const { return (int32_t)s[idx(y, x)]; }
     __device__ __forceinline__ void set(int32_t y, int32_t x, int32_t v) const {
-        if (y < 0 || x < 0 || y >= rows || x >= cols) return;
-        s[(y - ry0) * kLfRegion + (x - rx0)] = (L)v;
+        if (EDGE && (y < 0 || x < 0 || y >= rows || x >= cols)) return;
+        s[(y - ry0) * kLfStride + (x - rx0)] = (L)v;
     }
 };
 
 __device__ __forceinline__ int32_t f4clamp(int32_t v, int32_t lo, int32_t hi) { return clip3(lo, hi, v); }
 
+// wide_filter (loop_filter.rs:789-822) with N = n, N2 = n2, LOG2 = log2Size
+// fixed at compile time: output i in [-N, N) is
+// Round2(sum_{j=-N..N} v[Clip3(-(N+1), N, i+j)] * (|j| <= N2 ? 2 : 1), LOG2).
+template <int N, int N2, int LOG2, typename Reg>
+__device__ __forceinline__ void lf_wide(const Reg &R, const int32_t *v, int32_t x, int32_t y, int32_t dx,
+                                        int32_t dy) {
+    int32_t out[2 * N];
+#pragma unroll
+    for (int i = -N; i < N; i++) {
+        int32_t t = 0;
+#pragma unroll
+        for (int j = -N; j <= N; j++) {
+            const int pidx = i + j < -(N + 1) ? -(N + 1) : (i + j > N ? N : i + j);
+            t += v[pidx + 7] * ((j >= -N2 && j <= N2) ? 2 : 1);
+        }
+        out[i + N] = (t + (1 << (LOG2 - 1))) >> LOG2;
+    }
+#pragma unroll
+    for (int i = -N; i < N; i++) R.set(y + i * dy, x + i * dx, out[i + N]);
+}
+
 // sample_filtering + filter_mask + narrow/wide filter (loop_filter.rs:582-822)
 // for one sample position of one edge.
-template <typename L>
-__device__ void lf_filter_sample(const LfRegion<L> &R, int32_t x, int32_t y, int32_t dx, int32_t dy,
+template <typename L, bool EDGE>
+__device__ void lf_filter_sample(const LfRegion<L, EDGE> &R, int32_t x, int32_t y, int32_t dx, int32_t dy,
                                  uint32_t plane, uint32_t fs, uint32_t lvl, uint32_t sharp, uint32_t bd) {
     // strength (loop_filter.rs:502-523)
     const int32_t shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
@@ -207,35 +235,47 @@ __device__ void lf_filter_sample(const LfRegion<L> &R, int32_t x, int32_t y, int
         }
         return;
     }
-    // wide_filter (loop_filter.rs:789-822); tap index p = Clip3(-(n+1), n, i+j)
-    const uint32_t log2 = (fs == 8 || !flat2) ? 3 : 4;
-    const int32_t n = log2 == 4 ? 6 : (plane == 0 ? 3 : 2);
-    const int32_t n2 = (log2 == 3 && plane == 0) ? 0 : 1;
-    // v[t] for t = -7..6 -> index t+7: p[6..0] then q[0..6]
-    int32_t v[14];
+    // wide_filter (loop_filter.rs:789-822): 13-tap (flat2, filter size 16),
+    // 7-tap luma or 5-tap chroma (filter size >= 8)
+    int32_t v[14];  // v[t + 7] for t = -7..6: p6..p0 then q0..q6
 #pragma unroll
     for (int k = 0; k < 7; k++) {
         v[6 - k] = p[k];
         v[7 + k] = q[k];
     }
-    int32_t out[12];
-#pragma unroll
-    for (int ii = 0; ii < 12; ii++) {
-        const int32_t i = ii - n;
-        if (i >= n) break;
-        int32_t t = 0;
-        for (int32_t j = -n; j <= n; j++) {
-            const int32_t pidx = clip3(-(n + 1), n, i + j);
-            const int32_t tap = abs(j) <= n2 ? 2 : 1;
-            t += v[pidx + 7] * tap;
-        }
-        out[ii] = (t + (1 << (log2 - 1))) >> log2;
+    if (fs >= 16 && flat2)
+        lf_wide<6, 1, 4>(R, v, x, y, dx, dy);
+    else if (plane == 0)
+        lf_wide<3, 0, 3>(R, v, x, y, dx, dy);
+    else
+        lf_wide<2, 1, 3>(R, v, x, y, dx, dy);
+}
+
+// Passes 0 and 1 over the loaded region (see lf_tile).
+template <typename L, bool EDGE>
+__device__ void lf_passes(const LfRegion<L, EDGE> &R, int32_t X0, int32_t Y0, uint32_t plane,
+                          const uint16_t *dec0, const uint16_t *dec1, uint32_t sharp, uint32_t bd) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // pass 0: vertical edges x every loaded row; consecutive lanes take
+    // consecutive rows of one edge (one decision per 4 lanes)
+    // (loops run a workgroup-uniform trip count; the work is predicated)
+    for (int base = 0; base < kLfRegion * kLfEdges; base += nt) {
+        const int i = base + tid;
+        const int e = i / kLfRegion, ry = i - e * kLfRegion;
+        const int32_t y = R.ry0 + ry;
+        const bool rowok = !EDGE || (y >= 0 && y < R.rows);
+        const uint32_t d = (i < kLfRegion * kLfEdges && rowok) ? dec0[(ry >> 2) * kLfEdges + e] : 0u;
+        if (d) lf_filter_sample(R, X0 - 4 + 4 * e, y, 1, 0, plane, d & 0xffu, d >> 8, sharp, bd);
     }
-#pragma unroll
-    for (int ii = 0; ii < 12; ii++) {
-        const int32_t i = ii - n;
-        if (i >= n) break;
-        R.set(y + i * dy, x + i * dx, out[ii]);
+    __syncthreads();
+    // pass 1: horizontal edges x tile columns (consecutive lanes, consecutive columns)
+    for (int base = 0; base < kLfEdges * kLfTile; base += nt) {
+        const int i = base + tid;
+        const int er = i / kLfTile, c = i - er * kLfTile;
+        const int32_t x = X0 + c;
+        const bool colok = !EDGE || x < R.cols;
+        const uint32_t d = (i < kLfEdges * kLfTile && colok) ? dec1[er * (kLfTile / 4) + (c >> 2)] : 0u;
+        if (d) lf_filter_sample(R, x, Y0 - 4 + 4 * er, 0, 1, plane, d & 0xffu, d >> 8, sharp, bd);
     }
 }
 
@@ -247,57 +287,60 @@ __device__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t
     using L = typename LdsOf<T>::type;
     const int32_t rows = (int32_t)J.pad_rows[plane], cols = (int32_t)J.pad_cols[plane];
     const int32_t Y0 = (int32_t)ty * kLfTile, X0 = (int32_t)tx * kLfTile;
-    LfRegion<L> R{sm, Y0 - kLfHalo, X0 - kLfHalo, rows, cols};
+    const int32_t ry0 = Y0 - kLfHalo, rx0 = X0 - kLfHalo;
+    const bool interior = ry0 >= 0 && rx0 >= 0 && ry0 + kLfRegion <= rows && rx0 + kLfRegion <= cols;
     const int tid = threadIdx.x, nt = blockDim.x;
 
     // 1. load the in-plane part of the region
-    for (int i = tid; i < kLfRegion * kLfRegion; i += nt) {
-        const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
-        const int32_t y = R.ry0 + ry, x = R.rx0 + rx;
-        if (y >= 0 && y < rows && x >= 0 && x < cols) sm[i] = (L)src[(size_t)y * spitch + x];
+    // (every loop below runs a workgroup-uniform trip count; work is predicated)
+    if (interior) {
+        const G<T> *s0 = src + (size_t)ry0 * spitch + rx0;
+#pragma unroll 4
+        for (int base = 0; base < kLfRegion * kLfRegion; base += nt) {
+            const int i = base + tid;
+            const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
+            if (i < kLfRegion * kLfRegion) sm[ry * kLfStride + rx] = (L)s0[(size_t)ry * spitch + rx];
+        }
+    } else {
+#pragma unroll 4
+        for (int base = 0; base < kLfRegion * kLfRegion; base += nt) {
+            const int i = base + tid;
+            const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
+            const int32_t y = ry0 + ry, x = rx0 + rx;
+            if (i < kLfRegion * kLfRegion && y >= 0 && y < rows && x >= 0 && x < cols)
+                sm[ry * kLfStride + rx] = (L)src[(size_t)y * spitch + x];
+        }
     }
     // 2. edge decisions: pass 0 for region cell rows x edges X0-4..X0+TW+4,
     //    pass 1 for edge rows Y0-4..Y0+TH+4 x tile cell columns.
-    const uint32_t pass_dis = (J.apply_lf && J.lf_plane_on[plane]) ? 0u : 1u;
-    for (int i = tid; i < kLfCellRows * kLfEdges; i += nt) {
+    const bool lf_on = J.apply_lf && J.lf_plane_on[plane];
+    for (int base = 0; base < kLfCellRows * kLfEdges; base += nt) {
+        const int i = base + tid;
         const int cr = i / kLfEdges, e = i - cr * kLfEdges;
-        const int32_t by = (R.ry0 >> 2) + cr, bx = ((X0 - 4) >> 2) + e;
-        dec0[i] = pass_dis ? 0 : (uint16_t)lf_decide(J, plane, 0, by, bx);
+        const int32_t by = (ry0 >> 2) + cr, bx = ((X0 - 4) >> 2) + e;
+        if (i < kLfCellRows * kLfEdges) dec0[i] = lf_on ? (uint16_t)lf_decide(J, plane, 0, by, bx) : 0;
     }
-    for (int i = tid; i < kLfEdges * (kLfTile / 4); i += nt) {
+    for (int base = 0; base < kLfEdges * (kLfTile / 4); base += nt) {
+        const int i = base + tid;
         const int er = i / (kLfTile / 4), c = i - er * (kLfTile / 4);
         const int32_t by = ((Y0 - 4) >> 2) + er, bx = (X0 >> 2) + c;
-        dec1[i] = pass_dis ? 0 : (uint16_t)lf_decide(J, plane, 1, by, bx);
+        if (i < kLfEdges * (kLfTile / 4)) dec1[i] = lf_on ? (uint16_t)lf_decide(J, plane, 1, by, bx) : 0;
     }
     __syncthreads();
     const uint32_t sharp = J.lf_sharp, bd = J.bd;
-    // 3. pass 0: every loaded row x vertical edge
-    for (int i = tid; i < kLfRegion * kLfEdges; i += nt) {
-        const int ry = i / kLfEdges, e = i - ry * kLfEdges;
-        const int32_t y = R.ry0 + ry;
-        if (y < 0 || y >= rows) continue;
-        const uint32_t d = dec0[(ry >> 2) * kLfEdges + e];
-        if (!d) continue;
-        const int32_t x = X0 - 4 + 4 * e;
-        lf_filter_sample(R, x, y, 1, 0, plane, d & 0xffu, d >> 8, sharp, bd);
-    }
-    __syncthreads();
-    // 4. pass 1: horizontal edges x tile columns
-    for (int i = tid; i < kLfEdges * kLfTile; i += nt) {
-        const int er = i / kLfTile, c = i - er * kLfTile;
-        const int32_t x = X0 + c;
-        if (x >= cols) continue;
-        const uint32_t d = dec1[er * (kLfTile / 4) + (c >> 2)];
-        if (!d) continue;
-        const int32_t y = Y0 - 4 + 4 * er;
-        lf_filter_sample(R, x, y, 0, 1, plane, d & 0xffu, d >> 8, sharp, bd);
-    }
+    // 3./4. pass 0 and pass 1
+    if (interior)
+        lf_passes(LfRegion<L, false>{sm, ry0, rx0, rows, cols}, X0, Y0, plane, dec0, dec1, sharp, bd);
+    else
+        lf_passes(LfRegion<L, true>{sm, ry0, rx0, rows, cols}, X0, Y0, plane, dec0, dec1, sharp, bd);
     __syncthreads();
     // 5. store the tile
-    for (int i = tid; i < kLfTile * kLfTile; i += nt) {
-        const int yy = i / kLfTile, xx = i - yy * kLfTile;
+    for (int base = 0; base < kLfTile * kLfTile; base += nt) {
+        const int i = base + tid;
+        const int yy = i >> 6, xx = i & 63;
         const int32_t y = Y0 + yy, x = X0 + xx;
-        if (y < rows && x < cols) dst[(size_t)y * dpitch + x] = (T)sm[(y - R.ry0) * kLfRegion + (x - R.rx0)];
+        if (i < kLfTile * kLfTile && y < rows && x < cols)
+            dst[(size_t)y * dpitch + x] = (T)sm[(yy + kLfHalo) * kLfStride + (xx + kLfHalo)];
     }
 }
 

@@ -8,4 +8,4 @@ this package only binds it with ctypes for tests, benchmarks and tools:
   engine   the device-resident chain engine and HBM buffers (library allocator)
 """
 from . import abi  # noqa: F401
-from .frames import Frame, planes_struct  # noqa: F401
+from .frames import Frame, planes_struct, workload_seeds  # noqa: F401

@@ -33,6 +33,20 @@ def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct)) if a is not None else C.POINTER(ct)()
 
 
+def workload_seeds(cfg, n):
+    """Seeds of the n synthetic frames of benchmark configuration cfg:
+    splitmix64((0x0A1F0000 + cfg) ^ frame_index).  Configuration 3 takes
+    its resolution from seed bits 8-9, which are set so the n frames come
+    in four equal blocks of 640x360, 1280x720, 1920x1080 and 3840x2160."""
+    lib = abi.load()
+    base = 0x0A1F0000 + cfg
+    seeds = [int(lib.oxav1_gen_splitmix64(base ^ i)) for i in range(n)]
+    if cfg == 3:
+        block = max(1, (n + 3) // 4)
+        seeds = [(s & ~(3 << 8)) | (min(i // block, 3) << 8) for i, s in enumerate(seeds)]
+    return seeds
+
+
 class Frame:
     """One generated frame: padded pre-filter planes + grids + parameters."""
 
