@@ -5,8 +5,9 @@ Deblocking: the reference walks edges sequentially and in place
 (loop_filter.rs:271-294).  The HIP kernels filter every edge of a pass at
 once, which is the same computation only when no edge reads samples another
 edge of the same pass writes (true for legal, size-aligned block and
-transform geometry).  The oracle's walk is run both ways on generated frames
-and must agree; on a transform grid that is not size-aligned the two must
+transform geometry).  The oracle's walk is run forward, with every edge of a
+pass reading the pass-start samples, and in reverse order on generated
+frames, and all three must agree; on a transform grid that is not size-aligned the two must
 differ, which shows the comparison can fail.
 """
 import ctypes as C
@@ -27,14 +28,22 @@ def _frame(seed, **kw):
     return Frame.generate(gen_config(**kw), seed)
 
 
-def _lf_both(oracle_mod, f):
+def _lf_walks(oracle_mod, f):
+    """(forward, all-edges-from-snapshot, reversed) deblocked planes."""
     from oxav1.frames import planes_struct
     lib = oracle_mod.load()
     ctx, g, keep = f.lf_args()
-    seq = [p.copy() for p in f.planes]
-    par = [p.copy() for p in f.planes]
-    lib.oo_loop_filter_frame(C.byref(ctx), C.byref(g), planes_struct(seq), len(seq))
-    lib.oo_loop_filter_frame_parallel(C.byref(ctx), C.byref(g), planes_struct(par), len(par))
+    out = []
+    for fn in (lib.oo_loop_filter_frame, lib.oo_loop_filter_frame_parallel,
+               lib.oo_loop_filter_frame_reversed):
+        pl = [p.copy() for p in f.planes]
+        fn(C.byref(ctx), C.byref(g), planes_struct(pl), len(pl))
+        out.append(pl)
+    return out
+
+
+def _lf_both(oracle_mod, f):
+    seq, par, _ = _lf_walks(oracle_mod, f)
     return seq, par
 
 
@@ -46,10 +55,11 @@ def test_deblock_sequential_equals_parallel(fi, si, oracle_mod):
                bit_depth=bd, subsampling_x=sx, subsampling_y=sy, mono=mono,
                noise_amp=8 << (bd - 8), exercise_segments=si % 2, exercise_delta_lf=fi % 2,
                sb128=si % 2)
-    seq, par = _lf_both(oracle_mod, f)
+    seq, par, rev = _lf_walks(oracle_mod, f)
     assert any(not np.array_equal(a, b) for a, b in zip(seq, f.planes)), "nothing was filtered"
-    for p, (a, b) in enumerate(zip(seq, par)):
-        assert np.array_equal(a, b), f"plane {p}: {(a != b).sum()} samples differ"
+    for p, (a, b, c) in enumerate(zip(seq, par, rev)):
+        assert np.array_equal(a, b), f"plane {p}: parallel walk, {(a != b).sum()} samples differ"
+        assert np.array_equal(a, c), f"plane {p}: reversed walk, {(a != c).sum()} samples differ"
 
 
 def test_deblock_parallel_differs_on_overlapping_edges(oracle_mod):
