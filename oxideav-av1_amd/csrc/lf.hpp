@@ -293,22 +293,28 @@ __device__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t
 
     // 1. load the in-plane part of the region
     // (every loop below runs a workgroup-uniform trip count; work is predicated)
-    if (interior) {
-        const G<T> *s0 = src + (size_t)ry0 * spitch + rx0;
-#pragma unroll 4
-        for (int base = 0; base < kLfRegion * kLfRegion; base += nt) {
-            const int i = base + tid;
-            const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
-            if (i < kLfRegion * kLfRegion) sm[ry * kLfStride + rx] = (L)s0[(size_t)ry * spitch + rx];
-        }
-    } else {
-#pragma unroll 4
-        for (int base = 0; base < kLfRegion * kLfRegion; base += nt) {
-            const int i = base + tid;
-            const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
-            const int32_t y = ry0 + ry, x = rx0 + rx;
-            if (i < kLfRegion * kLfRegion && y >= 0 && y < rows && x >= 0 && x < cols)
-                sm[ry * kLfStride + rx] = (L)src[(size_t)y * spitch + x];
+    // In rounds of kRound loads per thread, every load of a round issued (at
+    // a clamped, in-plane index) before its LDS stores, so latencies overlap.
+    {
+        constexpr int kN = kLfRegion * kLfRegion, kRound = 8;
+        static_assert(4 * kRound * 256 >= kN, "region fits four rounds");
+#pragma unroll 1
+        for (int r0 = 0; r0 < 4 * kRound; r0 += kRound) {
+            L v[kRound];
+#pragma unroll
+            for (int k = 0; k < kRound; k++) {
+                const int i = min((r0 + k) * nt + tid, kN - 1);
+                const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
+                const int32_t y = clip3(0, rows - 1, ry0 + ry), x = clip3(0, cols - 1, rx0 + rx);
+                v[k] = (L)src[(size_t)y * spitch + x];
+            }
+#pragma unroll
+            for (int k = 0; k < kRound; k++) {
+                const int i = (r0 + k) * nt + tid;
+                const int ry = i / kLfRegion, rx = i - ry * kLfRegion;
+                const int32_t y = ry0 + ry, x = rx0 + rx;
+                if (i < kN && (interior || (y >= 0 && y < rows && x >= 0 && x < cols))) sm[ry * kLfStride + rx] = v[k];
+            }
         }
     }
     // 2. edge decisions: pass 0 for region cell rows x edges X0-4..X0+TW+4,
