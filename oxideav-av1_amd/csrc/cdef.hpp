@@ -194,8 +194,8 @@ __device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox
             int32_t sum = 0, mx = x, mn = x;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-                const int32_t thr = i < 2 ? b.pri : b.sec;
-                const int32_t adj = i < 2 ? b.pri_adj : b.sec_adj;
+                const uint32_t thr = (uint32_t)(i < 2 ? b.pri : b.sec);
+                const uint32_t adj = i < 2 ? b.pri_adj : b.sec_adj;
                 const int32_t w = i == 0 ? b.pt0 : (i == 1 ? b.pt1 : (i < 4 ? 2 : 1));
 #pragma unroll
                 for (int sg = 0; sg < 2; sg++) {
@@ -204,7 +204,16 @@ __device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox
                         if (!(py + dy >= 0 && py + dy < ph && px + dx >= 0 && px + dx < pw)) continue;
                     }
                     const int32_t v = (int32_t)c[sg ? -b.off[i] : b.off[i]];
-                    sum += w * cdef_constrain(v - x, thr, adj);
+                    // constrain (cdef.rs:592-606): sign(d) * min(|d|, t) with
+                    // t = max(0, thr - (|d| >> adj)) is d clamped to [-t, t]
+                    const int32_t d = v - x;
+                    uint32_t ad;
+                    if constexpr (std::is_same<L, uint16_t>::value)
+                        ad = __builtin_amdgcn_sad_u16((uint32_t)v, (uint32_t)x, 0u);  // samples < 2^16
+                    else
+                        ad = (uint32_t)abs(d);
+                    const int32_t tt = (int32_t)__builtin_elementwise_sub_sat(thr, ad >> adj);
+                    sum += w * max(min(d, tt), -tt);
                     mx = max(mx, v);
                     mn = min(mn, v);
                 }

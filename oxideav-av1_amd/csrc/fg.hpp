@@ -402,6 +402,11 @@ __device__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int
     const int32_t hw_old = psx ? 23 : (j == 0 ? 27 : 17), hw_new = psx ? 22 : (j == 0 ? 17 : 27);
     const G<int16_t> *pc = block_base(st, b) + j;
     const G<int16_t> *pl = hov ? block_base(st, b - 1) + j + (1 << bsh) : pc;
+    auto hval = [&](const G<int16_t> *c, const G<int16_t> *l, int32_t i) -> int32_t {
+        int32_t g = c[i * tw];
+        if (hov) g = clip3(gmin, gmax, fg_round2(l[i * tw] * hw_old + g * hw_new, 5));
+        return g;
+    };
     // vertical overlap: the first 2 (1) rows of a stripe blend with rows 32,
     // 33 (16) of the previous stripe's noise
     const int32_t vrows = (overlap && st > 0) ? (psy ? 1 : 2) : 0;
@@ -419,62 +424,31 @@ __device__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int
     const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
     const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) << (bd - 8);
     const int32_t maxs = (1 << bd) - 1;
-    // Rows go in groups of kFgRows: every load of a group (pre-grain sample,
-    // template taps, co-located luma) is issued before any of them is used,
-    // so their latencies overlap.
-    constexpr int kFgRows = 4;
-    const int32_t hw = 1 << ssh;  // rows of a noise stripe
-    for (int32_t y = yb0, i = ib; y < yb1; y += kFgRows, i += kFgRows) {
-        const int nr = min(kFgRows, yb1 - y);
-        int32_t o[kFgRows], g[kFgRows], gl[kFgRows], gu[kFgRows], gul[kFgRows], la[kFgRows], lb[kFgRows];
-#pragma unroll
-        for (int r = 0; r < kFgRows; r++) {
-            o[r] = g[r] = gl[r] = gu[r] = gul[r] = la[r] = lb[r] = 0;
-            if (r < nr) {
-                o[r] = (int32_t)src[(size_t)(y - yb0 + r) * ip];
-                g[r] = pc[(i + r) * tw];
-                if (hov) gl[r] = pl[(i + r) * tw];
-                if (i + r < vrows) {
-                    gu[r] = pu[(i + r + hw) * tw];
-                    if (hov) gul[r] = pul[(i + r + hw) * tw];
-                }
-                if (plane != 0) {
-                    const G<T> *lrow = ly + (size_t)((y + r) << psy) * lp;
-                    la[r] = (int32_t)lrow[lx];
-                    if (psx) lb[r] = (int32_t)lrow[lnx];
-                }
-            }
+    for (int32_t y = yb0, i = ib; y < yb1; y++, i++, src += ip, dst += op) {
+        int32_t nz = hval(pc, pl, i);
+        if (i < vrows) {
+            const int32_t old = hval(pu, pul, i + (1 << ssh));
+            const int32_t wo = psy ? 23 : (i == 0 ? 27 : 17), wn = psy ? 22 : (i == 0 ? 17 : 27);
+            nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
         }
-#pragma unroll
-        for (int r = 0; r < kFgRows; r++) {
-            if (r >= nr) break;
-            // noise: horizontal overlap blend, then vertical (film_grain.rs:705-739)
-            int32_t nz = hov ? clip3(gmin, gmax, fg_round2(gl[r] * hw_old + g[r] * hw_new, 5)) : g[r];
-            if (i + r < vrows) {
-                const int32_t old =
-                    hov ? clip3(gmin, gmax, fg_round2(gul[r] * hw_old + gu[r] * hw_new, 5)) : gu[r];
-                const int32_t wo = psy ? 23 : (i + r == 0 ? 27 : 17), wn = psy ? 22 : (i + r == 0 ? 17 : 27);
-                nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
-            }
-            const int32_t orig = o[r];
-            G<T> *d = dst + (size_t)(y - yb0 + r) * op;
-            if (plane == 0) {
-                const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
-                *d = (T)clip3(minv, maxl, orig + n2);
-                continue;
-            }
-            // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
-            const int32_t avg = psx ? fg_round2(la[r] + lb[r], 1) : la[r];
-            int32_t merged;
-            if (from_luma) {
-                merged = avg;
-            } else {
-                const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
-                merged = clip3(0, maxs, (comb >> 6) + coff);
-            }
-            const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
-            *d = (T)clip3(minv, maxc, orig + n2);
+        const int32_t orig = (int32_t)*src;
+        if (plane == 0) {
+            const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
+            *dst = (T)clip3(minv, maxl, orig + n2);
+            continue;
         }
+        // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
+        const G<T> *lrow = ly + (size_t)(y << psy) * lp;
+        const int32_t avg = psx ? fg_round2((int32_t)lrow[lx] + (int32_t)lrow[lnx], 1) : (int32_t)lrow[lx];
+        int32_t merged;
+        if (from_luma) {
+            merged = avg;
+        } else {
+            const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
+            merged = clip3(0, maxs, (comb >> 6) + coff);
+        }
+        const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
+        *dst = (T)clip3(minv, maxc, orig + n2);
     }
 }
 

@@ -45,23 +45,24 @@ __device__ void sr_rows(const G<T> *__restrict__ in, uint32_t ipitch, int32_t in
             f[k] = c_upscale_filter[ph][k];
             col[k] = clip3(0, last, clip3(0, maxx, px + (k - 3)));
         }
-        const bool interior = px - 3 >= 0 && px + 4 <= last;
-        const G<T> *rin = in + (size_t)ya * ipitch;
+        // rows in groups of four: the 32 tap loads of a group are
+        // unconditional (rows clamped into the item) so they overlap
         G<T> *rout = out + (size_t)ya * opitch + x;
-        if (interior) {
-            const G<T> *r0 = rin + (px - 3);
-            for (int32_t y = ya; y < yb; y++, r0 += ipitch, rout += opitch) {
-                int32_t sum = 0;
+        for (int32_t y = ya; y < yb; y += 4, rout += 4 * opitch) {
+            int32_t v[4][8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) sum += (int32_t)r0[k] * f[k];
-                *rout = (T)clip3(0, maxv, (sum + 64) >> 7);
+            for (int r = 0; r < 4; r++) {
+                const G<T> *row = in + (size_t)min(y + r, yb - 1) * ipitch;
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[r][k] = (int32_t)row[col[k]];
             }
-        } else {
-            for (int32_t y = ya; y < yb; y++, rin += ipitch, rout += opitch) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (y + r >= yb) break;
                 int32_t sum = 0;
 #pragma unroll
-                for (int k = 0; k < 8; k++) sum += (int32_t)rin[col[k]] * f[k];
-                *rout = (T)clip3(0, maxv, (sum + 64) >> 7);
+                for (int k = 0; k < 8; k++) sum += v[r][k] * f[k];
+                rout[(size_t)r * opitch] = (T)clip3(0, maxv, (sum + 64) >> 7);
             }
         }
     }
