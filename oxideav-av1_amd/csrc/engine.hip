@@ -113,8 +113,10 @@ template <typename T>
 __device__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
     const uint32_t p = plane_of(J.item_base[kStageLr], 3, local);
     const uint32_t t = local - J.item_base[kStageLr][p];
-    const uint32_t stripe = t / J.lr_tiles_x[p], tile = t - stripe * J.lr_tiles_x[p];
-    lr_tile<T>(J, p, stripe, tile, S, plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]);
+    const uint32_t per_stripe = J.lr_tiles_x[p] * lr_chunks(p ? J.sy : 0);
+    const uint32_t stripe = t / per_stripe, rest = t - stripe * per_stripe;
+    const uint32_t chunk = rest / J.lr_tiles_x[p], tile = rest - chunk * J.lr_tiles_x[p];
+    lr_tile<T>(J, p, stripe, chunk, tile, S, plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]);
 }
 
 template <typename T>
@@ -591,7 +593,7 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         if (p < np) {
             const uint32_t psy = p ? sy : 0;
             J.lr_tiles_x[p] = ceil_div(J.out_cols[p], J.lr_tile_w[p]);
-            acc += lr_stripe_count(J.out_rows[p] - 1, psy) * J.lr_tiles_x[p];
+            acc += lr_stripe_count(J.out_rows[p] - 1, psy) * lr_chunks(psy) * J.lr_tiles_x[p];
         }
     }
     J.item_base[kStageLr][3] = acc;

@@ -1,5 +1,6 @@
 // §7.17 loop restoration on gfx950: one workgroup per (plane, 64-luma-row
-// stripe, column tile), Wiener or self-guided chosen per tile.
+// stripe, 32-row chunk of the stripe, column tile), Wiener or self-guided
+// chosen per tile.
 //
 // Reference: loop_restoration.rs:268-992.  The reference filters 4x4 blocks
 // one at a time and fetches every tap through get_source_sample
@@ -14,7 +15,9 @@
 // are built per stripe over [stripe-1, stripe+1] with the stripe's routing
 // (the reference computes each block's A/B halo with that block's routing,
 // which is the same map); pass 0 only needs odd rows (loop_restoration.rs
-// :686-699 weights even rows 0, and block rows start even).
+// :686-699 weights even rows 0, and block rows start even).  A luma stripe is
+// split into two 32-row chunks (the routing still follows the stripe), which
+// halves the LDS footprint so twice as many workgroups fit on a CU.
 #pragma once
 
 #include "common.hpp"
@@ -22,7 +25,7 @@
 namespace oxav1 {
 
 constexpr int kLrMaxW = 64;                 // max tile width (plane samples)
-constexpr int kLrMaxH = 64;                 // max stripe height
+constexpr int kLrMaxH = 32;                 // max rows of one chunk of a stripe
 constexpr int kLrSw = kLrMaxW + 6;          // source window width
 constexpr int kLrSh = kLrMaxH + 6;          // source window height
 constexpr int kLrAw = kLrMaxW + 2;          // A/B columns: tile cols -1 .. W
@@ -35,11 +38,20 @@ __constant__ uint16_t c_sgr_a2[256];        // a2 for 0 < z < 255 (loop_restorat
 // A (<= 256, 9 bits) and B (< 2^20) of one self-guided position in one word.
 __device__ __forceinline__ uint32_t lr_pack_ab(uint32_t a, uint32_t b) { return (a << 20) | b; }
 
+// Chunks per stripe: a luma stripe (64 rows) has two, a vertically
+// subsampled chroma stripe (32 rows) one.
+constexpr uint32_t lr_chunks(uint32_t sy) { return (64u >> sy) / kLrMaxH; }
+
 template <typename L>
 struct LrSmem {
     L src[kLrSh * kLrSw];              // routed source window, rows ys-3 .. ye+3
-    uint32_t ab1[kLrAh1 * kLrAw];      // pass-1 (A, B)
-    uint32_t ab0[kLrAh0 * kLrAw];      // pass-0 (A, B), odd rows
+    union {
+        struct {
+            uint32_t ab1[kLrAh1 * kLrAw];  // pass-1 (A, B)
+            uint32_t ab0[kLrAh0 * kLrAw];  // pass-0 (A, B), odd rows
+        };
+        int32_t inter[kLrSh * kLrMaxW];    // Wiener horizontal pass
+    };
     uint64_t rowp[kLrSh];              // routed base address of each window row
     int32_t rowmax[kLrSh];             // last column of that row's buffer; -1: none
     uint16_t a2[256];                  // c_sgr_a2 staged for lane-indexed reads
@@ -82,7 +94,7 @@ __device__ __forceinline__ void lr_sgr_consts(int32_t r, int32_t e, int32_t *n, 
 // One tile: (plane, stripe, column tile) -> out (cropped frame coordinates
 // equal plane coordinates).  Writes rows [ys, ye] x cols [xs, xe].
 template <typename T>
-__device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t tile,
+__device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
                         LrSmem<typename LdsOf<T>::type> &S, G<T> *__restrict__ out, uint32_t opitch) {
     using L = typename LdsOf<T>::type;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
@@ -102,7 +114,7 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
     if (g.ys > g.ye || g.xs > g.xe) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wv = tid >> 6, nwv = nt >> 6;
-    const int32_t H = g.ye - g.ys + 1, W = g.xe - g.xs + 1;
+    const int32_t W = g.xe - g.xs + 1;
     const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
     const uint32_t dpitch = J.up_cdef_pitch[plane];
 
@@ -116,6 +128,11 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
         u = J.lr_units[plane] ? J.lr_units[plane][(size_t)ur * J.lr_unit_cols[plane] + uc] : oxav1_lr_unit{};
         type = u.type;
     }
+    // this chunk's rows of the stripe
+    g.ys += (int32_t)chunk * kLrMaxH;
+    g.ye = min(g.ye, g.ys + kLrMaxH - 1);
+    if (g.ys > g.ye) return;
+    const int32_t H = g.ye - g.ys + 1;
     // Output rows are split into one band per wave; a lane owns a column.
     const int RB = (H + nwv - 1) / nwv;
     const int b0 = wv * RB, b1 = min(H, b0 + RB);
@@ -190,19 +207,26 @@ __device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint
         const uint32_t r0 = bd == 12 ? 5 : 3, r1 = bd == 12 ? 9 : 11;
         const int32_t offset = 1 << (bd + 7 - (int32_t)r0 - 1);
         const int32_t limit = (1 << (bd + 1 + 7 - (int32_t)r0)) - 1;
+        // horizontal pass: window rows 0 .. H+5 (tile rows -3 .. H+2) into
+        // LDS, one row per wave at a time, lanes on columns
+        for (int rb = 0; rb < H + 6; rb += nwv) {
+            const int wr = rb + wv;
+            if (wr < H + 6 && lane < W) {
+                const L *q = S.src + wr * kLrSw + lane;
+                const int32_t acc = hf[0] * ((int32_t)q[0] + (int32_t)q[6]) + hf[1] * ((int32_t)q[1] + (int32_t)q[5]) +
+                                    hf[2] * ((int32_t)q[2] + (int32_t)q[4]) + hf[3] * (int32_t)q[3];
+                S.inter[wr * kLrMaxW + lane] = clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
+            }
+        }
+        __syncthreads();
         if (lane >= W || b0 >= b1) return;
-        // horizontal pass of window row wr at this lane's column
-        auto hrow = [&](int wr) -> int32_t {
-            const L *q = S.src + wr * kLrSw + lane;
-            const int32_t acc = hf[0] * ((int32_t)q[0] + (int32_t)q[6]) + hf[1] * ((int32_t)q[1] + (int32_t)q[5]) +
-                                hf[2] * ((int32_t)q[2] + (int32_t)q[4]) + hf[3] * (int32_t)q[3];
-            return clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
-        };
-        int32_t h0 = hrow(b0), h1 = hrow(b0 + 1), h2 = hrow(b0 + 2), h3 = hrow(b0 + 3), h4 = hrow(b0 + 4),
-                h5 = hrow(b0 + 5);
+        // vertical pass down the wave's band with a 7-row register window
+        const int32_t *iv = S.inter + lane;
+        int32_t h0 = iv[b0 * kLrMaxW], h1 = iv[(b0 + 1) * kLrMaxW], h2 = iv[(b0 + 2) * kLrMaxW],
+                h3 = iv[(b0 + 3) * kLrMaxW], h4 = iv[(b0 + 4) * kLrMaxW], h5 = iv[(b0 + 5) * kLrMaxW];
         G<T> *o = out + (size_t)(g.ys + b0) * opitch + (g.xs + lane);
         for (int r = b0; r < b1; r++, o += opitch) {
-            const int32_t h6 = hrow(r + 6);
+            const int32_t h6 = iv[(r + 6) * kLrMaxW];
             const int32_t acc = vf[0] * (h0 + h6) + vf[1] * (h1 + h5) + vf[2] * (h2 + h4) + vf[3] * h3;
             *o = (T)clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1);
             h0 = h1, h1 = h2, h2 = h3, h3 = h4, h4 = h5, h5 = h6;
