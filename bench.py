@@ -9,6 +9,10 @@ grain with overlap) pushed through deblock -> CDEF -> superres -> loop
 restoration -> film grain.  Inputs (pre-filter planes, grids, parameters)
 are resident in HBM before the timed region.
 
+`--cfg 3 --frames 64` runs the mixed batch instead: 64 frames in four
+blocks of 640x360, 1280x720, 1920x1080 and 3840x2160, 8/10-bit and
+4:2:0/4:2:2/4:4:4 drawn per frame.
+
 value  = output Mpixel/s of the batched path (one persistent launch per
          stage over the whole batch, replayed from a hipGraph).
 plain  = the same frames, one launch per stage per frame.
@@ -28,6 +32,51 @@ sys.path.insert(0, os.path.join(ROOT, "oxideav-av1_amd"))
 
 METRIC = "byte-exact post-filter Mpixel/s at 4K 10-bit, batched vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
+
+
+def describe(frames):
+    """Short description of a list of frames: sizes, depths, subsampling."""
+    from collections import Counter
+    c = Counter()
+    for f in frames:
+        d = f.desc
+        ss = "mono" if d.num_planes == 1 else {(1, 1): "4:2:0", (1, 0): "4:2:2", (0, 0): "4:4:4"}[
+            (d.subsampling_x, d.subsampling_y)]
+        sr = f" coded {d.frame_width} wide" if d.upscaled_width > d.frame_width else ""
+        c[f"{d.upscaled_width}x{d.frame_height}{sr} {d.bit_depth}-bit {ss}"] += 1
+    return ", ".join(f"{n} x {k}" for k, n in c.items())
+
+
+def stages_on(frames):
+    names = {"lf": "deblock", "cdef": "CDEF", "sr": "superres", "lr": "loop restoration", "fg": "film grain"}
+    return "+".join(names[k] for k in stage_keys(frames))
+
+
+def stage_keys(frames):
+    """Engine stage names that do work on at least one of the frames."""
+    keys = ["lf", "cdef"]
+    if any(f.desc.use_superres and f.desc.upscaled_width > f.desc.frame_width for f in frames):
+        keys.append("sr")
+    if any(f.desc.lr.uses_lr for f in frames):
+        keys.append("lr")
+    if any(f.desc.fg.apply_grain for f in frames):
+        keys.append("fg")
+    return keys
+
+
+def measured_traffic(cfg, nframes, stage):
+    """HBM bytes per launch of the stage's kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a
+    copy kernel of known size); None when no summary covers it."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"traffic_cfg{cfg}_batch{nframes}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s+(\d+)", line.strip())
+        if m and m.group(1) == stage:
+            return int(m.group(2))
+    return None
 
 
 def main():
@@ -53,7 +102,7 @@ def main():
 
     import numpy as np
 
-    from oxav1 import Frame, abi
+    from oxav1 import Frame, abi, workload_seeds
     from oxav1.engine import DeviceFrame, Engine, device_info
     from oxav1.traffic import stage_bytes
 
@@ -77,15 +126,14 @@ def main():
         dst.free()
         return
 
-    base = 0x0A1F0000 + args.cfg
-    seeds = [int(lib.oxav1_gen_splitmix64(base ^ i)) for i in range(args.frames)]
+    seeds = workload_seeds(args.cfg, args.frames)
     t0 = time.time()
     frames = [Frame.generate(args.cfg, s) for s in seeds]
     dfs = [DeviceFrame(f) for f in frames]
     gen_s = time.time() - t0
     d0 = frames[0].desc
-    px_per_frame = d0.upscaled_width * d0.frame_height
-    total_px = px_per_frame * len(frames)
+    fpx = [f.desc.upscaled_width * f.desc.frame_height for f in frames]
+    total_px = sum(fpx)
 
     eng = Engine(max_frames=len(dfs))
     eng.prepare(dfs)
@@ -130,7 +178,8 @@ def main():
     for _ in range(max(3, args.steps // 3)):
         eng.run(1)
     eng.sync()
-    stage_ms = {s: eng.stage_ms(s)[0] for s in ("lf", "cdef", "sr", "lr", "fg")}
+    active = stage_keys(frames)
+    stage_ms = {s: eng.stage_ms(s)[0] for s in active}
     eng.set_timing(False)
     sb = {k: sum(stage_bytes(f.desc)[k] for f in frames) for k in stage_ms}
     dom = max(stage_ms, key=lambda k: stage_ms[k])
@@ -161,18 +210,20 @@ def main():
         import oracle
         oracle.load()
         n = min(args.cpu_frames, len(frames))
+        # evenly spaced over the batch, so a mixed batch's sample covers each of its blocks
+        pick = [i * len(frames) // n for i in range(n)]
         t = time.perf_counter()
-        outs = [oracle.chain(frames[i]) for i in range(n)]
+        outs = [oracle.chain(frames[i]) for i in pick]
         cpu_s = time.perf_counter() - t
         oracle_exact = 0
-        for i in range(n):
-            ref_o, out_o, _ = outs[i]
+        for i, (ref_o, out_o, _) in zip(pick, outs):
             ref_g, out_g = plain[i]
             oracle_exact += all(np.array_equal(a, b) for a, b in zip(ref_o + out_o, ref_g + out_g))
-        cpu = {"value": round(n * px_per_frame / cpu_s / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
+        which = f"the first {n}" if pick == list(range(n)) else f"{n} evenly spaced"
+        cpu = {"value": round(sum(fpx[i] for i in pick) / cpu_s / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
                "kind": "port",
-               "sample": f"{n} of the {len(frames)} frames (3840x2160 10-bit, full chain) through "
-                         f"the C oracle, single thread, {cpu_s:.1f} s"}
+               "sample": f"{which} of the {len(frames)} frames of the step ({describe([frames[i] for i in pick])}), "
+                         f"full chain through the C oracle, single thread, {cpu_s:.1f} s"}
 
     value = total_px / s_batch / 1e6
     line = {
@@ -186,12 +237,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16" if d0.bit_depth > 8 else "u8",
+        "dtype": "+".join(sorted({"u16" if f.desc.bit_depth > 8 else "u8" for f in frames}, reverse=True)),
         "data": "synthetic",
         "config": {
-            "workload": f"{len(frames)} x {d0.upscaled_width}x{d0.frame_height} {d0.bit_depth}-bit 4:2:0, "
-                        f"coded {d0.frame_width} wide (superres 12), 128x128 SB, LR 128x128 "
-                        f"Wiener/SGR, film grain overlap; deblock+CDEF+superres+LR+film grain",
+            "workload": describe(frames) + "; " + stages_on(frames),
             "frames_per_step": len(frames),
             "path": "batched persistent kernels, hipGraph replay",
         },
@@ -202,21 +251,22 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.cfg, len(frames), dom),
         },
         "cpu_baseline": cpu,
         "plain_mpx_s": round(total_px / s_plain / 1e6, 1),
         "batched_nograph_mpx_s": round(total_px / s_batch_nograph / 1e6, 1),
         "batched_vs_plain": round(s_plain / s_batch, 3),
         "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3),
-                       "what": "one 4K frame alone, plain path, enqueue to completion"},
+                       "what": f"one {d0.upscaled_width}x{d0.frame_height} frame alone, plain path, "
+                               f"enqueue to completion"},
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage_ms.items()},
         "stage_gbs": {k: round(sb[k] / (stage_ms[k] * 1e-3) / 1e9, 1) if stage_ms[k] else 0
                       for k in stage_ms},
         "chain_algorithmic_gbs": round(chain_gbs, 1),
         "chain_roofline_frac": round(chain_gbs / HBM_PEAK_GBS, 4),
         "byte_exact": {"batched_eq_plain_frames": exact_paths, "frames": len(frames),
-                       "oracle_checked": args.cpu_frames if cpu else 0,
+                       "oracle_checked": len(pick) if cpu else 0,
                        "oracle_exact": oracle_exact},
         "device": {"cus": num_cus, "hbm_gb": round(hbm_total / 2**30, 1)},
         "setup_s": round(gen_s, 2),
