@@ -47,6 +47,7 @@ extern "C" {
 #define OXAV1_ERR_ADMISSION_REFUSED 14
 #define OXAV1_ERR_UNSUPPORTED 15
 #define OXAV1_ERR_NO_DEVICE 16
+#define OXAV1_ERR_BUSY 17
 
 /* ---------------------------------------------------------------------- */
 /* Shared types                                                            */
@@ -395,12 +396,117 @@ int oxav1_pf_engine_run_batch(oxav1_pf_engine *e, const oxav1_frame_desc *desc,
 void *oxav1_pf_engine_stream(oxav1_pf_engine *e);
 /* Wait for all work on the engine's stream. */
 int oxav1_pf_engine_sync(oxav1_pf_engine *e);
+/* Launch shape knobs, chosen per filter and per sample size by the
+ * autotuner (oxav1/autotune.py) and applied from the next prepare.  Index
+ * [0] applies to 8-bit frames (batches), [1] to 10/12-bit.  Zero fields
+ * keep the defaults.  None of them changes any output byte. */
+typedef struct oxav1_tuning {
+    uint16_t blocks_per_cu[5][2]; /* persistent workgroups per CU for lf, cdef, sr, lr, fg
+                                     (0 = as many as fit); a batch uses the row of the
+                                     sample size holding most of its pixels */
+    uint8_t sr_rows_per_item[2];  /* superres rows per work item (0 = 8, at most 64) */
+    uint8_t lr_tile_w[2];         /* loop restoration tile width: 32, or 64 where it divides
+                                     the unit (0 = widest that divides) */
+} oxav1_tuning;
+int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning *t);
+/* Current tuning, the CU count and how many workgroups of each stage fit
+ * on one CU (any pointer may be NULL). */
+int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, uint32_t *num_cus,
+                               uint32_t fit_blocks_per_cu[5]);
+/* OXAV1_OK when everything enqueued on the engine's stream has finished,
+ * OXAV1_ERR_BUSY while work is in flight; never blocks. */
+int oxav1_pf_engine_query(oxav1_pf_engine *e);
+/* Size n frame slots for frames of desc's shape with every stage on, so
+ * that prepare with frames no larger never allocates; *bytes (optional)
+ * receives the engine's slot bytes. */
+int oxav1_pf_engine_reserve(oxav1_pf_engine *e, const oxav1_frame_desc *desc, uint32_t n,
+                            uint64_t *bytes);
 /* Time the dominant kernel of the last runs: average duration in ms of the
  * named stage ("lf","cdef","sr","lr","fg") measured with HIP events on the
  * engine's stream when timing is enabled. */
 int oxav1_pf_engine_set_timing(oxav1_pf_engine *e, int enable);
 int oxav1_pf_engine_stage_ms(oxav1_pf_engine *e, const char *stage, double *avg_ms,
                              uint32_t *count);
+
+/* ---------------------------------------------------------------------- */
+/* Serving loop: concurrent decode sessions, continuous batching           */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * A server owns `lanes` engines, each on its own stream, and one HBM arena
+ * from which sessions are admitted by footprint.  A session is a decode
+ * stream of frames of one shape; it owns `ref_slots` post-filter reference
+ * slots (the pre-grain frame store of frame_driver.rs:1044-1052, plus a
+ * film-grain output per slot when its frames carry grain) and an optional
+ * input region for its pre-filter frames, all resident in the arena until
+ * the session closes.  Frames are submitted as their pre-filter data becomes
+ * ready and wait in one FIFO; every poll retires finished lanes and hands the
+ * frames ready at that moment (up to max_batch) to a free lane as one
+ * batched launch per stage, so frames that arrive while others are on the
+ * device join the next free lane and lanes overlap on the device.
+ * One host thread drives a server (the calls are not thread-safe).
+ */
+typedef struct oxav1_pf_server oxav1_pf_server;
+
+typedef struct oxav1_server_config {
+    uint32_t lanes;              /* engines on their own streams (0 = 4) */
+    uint32_t max_batch;          /* frames per lane launch (0 = 16) */
+    uint64_t arena_bytes;        /* HBM for sessions (0 = half the free HBM) */
+    const oxav1_frame_desc *largest; /* lane intermediates sized for it (NULL = grow on demand) */
+} oxav1_server_config;
+
+typedef struct oxav1_server_done {
+    uint64_t ticket;
+    uint32_t session;
+    uint32_t slot;
+    int32_t status;              /* OXAV1_OK or the failing call's status */
+    uint32_t batch;              /* frames in the launch that carried it */
+    double arrival_us;           /* server clock (oxav1_pf_server_now_us) */
+    double launch_us;
+    double done_us;              /* when a poll saw the lane finish */
+} oxav1_server_done;
+
+typedef struct oxav1_server_stats {
+    uint32_t sessions_open;
+    uint32_t lanes_busy;
+    uint32_t frames_queued;
+    uint32_t max_batch_seen;
+    uint64_t frames_submitted;
+    uint64_t frames_done;
+    uint64_t launches;
+    uint64_t arena_bytes;
+    uint64_t arena_used;
+    uint64_t lane_bytes;         /* lane intermediates reserved at create */
+} oxav1_server_stats;
+
+double oxav1_pf_server_now_us(void);
+int oxav1_pf_server_create(oxav1_pf_server **out, const oxav1_server_config *cfg);
+/* Waits for every lane, then frees the engines and the arena. */
+int oxav1_pf_server_destroy(oxav1_pf_server *s);
+/* Admit a session whose frames have `shape` (sizes, depth, subsampling,
+ * superres and film-grain flags): reserves ref_slots reference slots plus
+ * input_bytes of input region in the arena.  OXAV1_ERR_ADMISSION_REFUSED
+ * when the footprint does not fit what the open sessions leave free;
+ * *input (optional) receives the input region's device pointer. */
+int oxav1_pf_server_open_session(oxav1_pf_server *s, const oxav1_frame_desc *shape, uint32_t ref_slots,
+                                 uint64_t input_bytes, uint32_t *session, void **input,
+                                 uint64_t *footprint);
+/* OXAV1_ERR_BUSY while a frame of the session is queued or in flight. */
+int oxav1_pf_server_close_session(oxav1_pf_server *s, uint32_t session);
+/* Device pointer and pitch (samples) of a slot's plane: which = 0 for the
+ * pre-grain reference frame, 1 for the film-grain output. */
+int oxav1_pf_server_slot(oxav1_pf_server *s, uint32_t session, uint32_t slot, uint32_t plane, int which,
+                         void **ptr, uint32_t *pitch);
+/* Queue one frame of a session; its outputs go to the session's slot
+ * (dev's ref_out/out are replaced by the slot's).  The frame's shape must
+ * match the session's.  OXAV1_ERR_BUSY when the slot still has a frame
+ * queued or in flight. */
+int oxav1_pf_server_submit(oxav1_pf_server *s, uint32_t session, const oxav1_frame_desc *desc,
+                           const oxav1_frame_dev *dev, uint32_t slot, double arrival_us, uint64_t *ticket);
+/* Retire finished lanes, launch ready frames on free lanes, and return up
+ * to cap finished frames (the rest are kept for the next poll). */
+int oxav1_pf_server_poll(oxav1_pf_server *s, oxav1_server_done *done, uint32_t cap, uint32_t *n_done);
+int oxav1_pf_server_stats(oxav1_pf_server *s, oxav1_server_stats *st);
 
 #ifdef __cplusplus
 }

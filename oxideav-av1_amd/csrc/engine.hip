@@ -348,9 +348,11 @@ static void sr_consts(uint32_t dw, uint32_t uw, uint32_t mi_cols, uint32_t sx, i
 }
 
 // LR tile width for a unit size: must divide it; stripes never straddle units.
-static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw) {
+// pref: 0 = the widest of 64/32 that divides the unit, else 32 or 64 when it divides.
+static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw, uint32_t pref = 0) {
     if (us < 32 || us % 32 != 0 || us < (64u >> sy)) return OXAV1_ERR_UNSUPPORTED;
     *tw = (us % 64 == 0) ? 64u : 32u;
+    if (pref == 32) *tw = 32;
     return OXAV1_OK;
 }
 
@@ -399,7 +401,9 @@ struct oxav1_pf_engine {
     hipEvent_t ev[8] = {};
     double stage_sum[6] = {0, 0, 0, 0, 0, 0};
     uint32_t stage_cnt[6] = {0, 0, 0, 0, 0, 0};
-    int persistent_blocks[5] = {0, 0, 0, 0, 0};
+    int persistent_blocks[5] = {0, 0, 0, 0, 0};  // workgroups per CU that fit, per stage
+    oxav1_tuning tuning = {};                       // zero = defaults
+    int batch_kind = 1;                             // 0 = mostly 8-bit, 1 = mostly 10/12-bit
 };
 
 static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
@@ -492,7 +496,10 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         }
     }
     // ---- superres ----
-    J.sr_rows_per_item = 8;
+    {
+        const uint32_t r = e->tuning.sr_rows_per_item[J.kind == kU8 ? 0 : 1];
+        J.sr_rows_per_item = r ? r : 8;
+    }
     if (J.use_sr) {
         for (uint32_t p = 0; p < np; p++) {
             const uint32_t psx = p ? sx : 0;
@@ -534,7 +541,7 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         J.lr_tile_w[p] = 64;
         if (J.lr_type[p] != OXAV1_RESTORE_NONE) {
             const uint32_t us = d.lr.loop_restoration_size[p];
-            int st = lr_tile_width(us, psy, &J.lr_tile_w[p]);
+            int st = lr_tile_width(us, psy, &J.lr_tile_w[p], e->tuning.lr_tile_w[J.kind == kU8 ? 0 : 1]);
             if (st) return st;
             J.lr_unit_size[p] = us;
             J.lr_unit_rows[p] = oxav1_pf_count_units_in_frame(us, rnd2(J.fh, psy));
@@ -638,7 +645,7 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[s], kThreads, 0) != hipSuccess ||
                 per_cu <= 0)
                 per_cu = 2;
-            e->persistent_blocks[s] = e->num_cus * per_cu;
+            e->persistent_blocks[s] = per_cu;
         }
     }
     e->max_frames = max_frames ? max_frames : 1;
@@ -705,6 +712,12 @@ extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_des
         e->any_fg |= e->jobs_h[f].apply_fg != 0;
     }
     e->n = n;
+    {
+        uint64_t px[2] = {0, 0};
+        for (uint32_t f = 0; f < n; f++)
+            px[e->jobs_h[f].kind == kU8 ? 0 : 1] += (uint64_t)e->jobs_h[f].uw * e->jobs_h[f].fh;
+        e->batch_kind = px[0] > px[1] ? 0 : 1;
+    }
     e->prefix_h.assign(5 * (n + 1), 0);
     for (int s = 0; s < 5; s++) {
         uint32_t acc = 0;
@@ -780,7 +793,10 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
         const uint32_t total = e->totals[s];
         if (s == kStageFg && e->any_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
         if (total) {
-            const uint32_t grid = std::min<uint32_t>(total, (uint32_t)e->persistent_blocks[s]);
+            const uint32_t per_cu = e->tuning.blocks_per_cu[s][e->batch_kind];
+            const uint32_t fit = (uint32_t)e->persistent_blocks[s];
+            const uint32_t grid =
+                std::min<uint32_t>(total, (uint32_t)e->num_cus * (per_cu ? std::min(per_cu, fit) : fit));
             hipLaunchKernelGGL(kEngineKernels[s], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
                                e->n, total, ctr + s);
             HIP_TRY(hipGetLastError());
@@ -874,6 +890,77 @@ extern "C" int oxav1_pf_engine_stage_ms(oxav1_pf_engine *e, const char *stage, d
 
 extern "C" void *oxav1_pf_engine_stream(oxav1_pf_engine *e) { return e ? (void *)e->stream : nullptr; }
 
+// Size n frame slots for frames of desc's shape with every stage on, so that
+// later prepare calls with frames no larger never allocate (a grow would
+// hipFree, which waits for the whole device).
+extern "C" int oxav1_pf_engine_reserve(oxav1_pf_engine *e, const oxav1_frame_desc *desc, uint32_t n,
+                                       uint64_t *bytes) {
+    if (!e || !desc || n == 0 || n > e->max_frames) return OXAV1_ERR_INVALID_ARG;
+    const oxav1_frame_desc &d = *desc;
+    if ((d.bit_depth != 8 && d.bit_depth != 10 && d.bit_depth != 12) || (d.num_planes != 1 && d.num_planes != 3) ||
+        d.mi_rows == 0 || d.mi_cols == 0 || d.frame_height == 0)
+        return OXAV1_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->slots.size() < n) e->slots.resize(n);
+    const uint32_t sx = d.subsampling_x ? 1 : 0, sy = d.subsampling_y ? 1 : 0;
+    const size_t B = d.bit_depth == 8 ? 1 : 2;
+    const uint32_t uw = std::max(d.upscaled_width, d.frame_width);
+    const bool sr = d.use_superres && uw > d.frame_width;
+    uint64_t total = 0;
+    for (uint32_t f = 0; f < n; f++) {
+        FrameSlot &slot = e->slots[f];
+        for (uint32_t p = 0; p < d.num_planes; p++) {
+            const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+            const size_t padn = (size_t)((d.mi_rows * 4) >> psy) * ((d.mi_cols * 4) >> psx) * B;
+            const size_t outn = (size_t)rnd2(d.frame_height, psy) * rnd2(uw, psx) * B;
+            int st = slot.deb[p].ensure(padn);
+            if (!st) st = slot.cdf[p].ensure(padn);
+            if (!st && sr) st = slot.upc[p].ensure(outn);
+            if (!st && sr) st = slot.upd[p].ensure(outn);
+            if (st) return st;
+        }
+        const uint32_t nst = ceil_div(ceil_div(d.frame_height, 2), 16), nbl = ceil_div(ceil_div(uw, 2), 16);
+        int st = slot.fgs.ensure(sizeof(FgScratch));
+        if (!st) st = slot.offs.ensure((size_t)nst * nbl);
+        if (st) return st;
+        for (uint32_t p = 0; p < 3; p++) total += slot.deb[p].n + slot.cdf[p].n + slot.upc[p].n + slot.upd[p].n;
+        total += slot.fgs.n + slot.offs.n;
+    }
+    if (bytes) *bytes = total;
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning *t) {
+    if (!e || !t) return OXAV1_ERR_INVALID_ARG;
+    for (int k = 0; k < 2; k++) {
+        if (t->sr_rows_per_item[k] > 64) return OXAV1_ERR_INVALID_ARG;
+        if (t->lr_tile_w[k] != 0 && t->lr_tile_w[k] != 32 && t->lr_tile_w[k] != 64) return OXAV1_ERR_INVALID_ARG;
+    }
+    e->tuning = *t;
+    return OXAV1_OK;
+}
+
+extern "C" int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, uint32_t *num_cus,
+                                          uint32_t fit_blocks_per_cu[5]) {
+    if (!e) return OXAV1_ERR_INVALID_ARG;
+    if (t) *t = e->tuning;
+    if (num_cus) *num_cus = (uint32_t)e->num_cus;
+    if (fit_blocks_per_cu)
+        for (int s = 0; s < 5; s++) fit_blocks_per_cu[s] = (uint32_t)e->persistent_blocks[s];
+    return OXAV1_OK;
+}
+
+// OXAV1_OK when all work enqueued on the engine's stream has finished,
+// OXAV1_ERR_BUSY while some is still running.  Does not block.
+extern "C" int oxav1_pf_engine_query(oxav1_pf_engine *e) {
+    if (!e) return OXAV1_ERR_INVALID_ARG;
+    const hipError_t q = hipStreamQuery(e->stream);
+    if (q == hipSuccess) return OXAV1_OK;
+    if (q == hipErrorNotReady) return OXAV1_ERR_BUSY;
+    return hip_fail(q, "hipStreamQuery");
+}
+
 // --------------------------------------------------------------------- //
 // Library basics                                                        //
 // --------------------------------------------------------------------- //
@@ -893,6 +980,7 @@ extern "C" const char *oxav1_pf_strerror(int status) {
     case OXAV1_ERR_ADMISSION_REFUSED: return "admission refused";
     case OXAV1_ERR_UNSUPPORTED: return "unsupported parameter combination";
     case OXAV1_ERR_NO_DEVICE: return "no HIP device";
+    case OXAV1_ERR_BUSY: return "busy: work still in flight";
     default: return "unknown status";
     }
 }
@@ -1010,6 +1098,10 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_film_grain_params") return sizeof(oxav1_film_grain_params);
     if (n == "oxav1_frame_desc") return sizeof(oxav1_frame_desc);
     if (n == "oxav1_frame_dev") return sizeof(oxav1_frame_dev);
+    if (n == "oxav1_server_config") return sizeof(oxav1_server_config);
+    if (n == "oxav1_tuning") return sizeof(oxav1_tuning);
+    if (n == "oxav1_server_done") return sizeof(oxav1_server_done);
+    if (n == "oxav1_server_stats") return sizeof(oxav1_server_stats);
     return 0;
 }
 

@@ -22,6 +22,7 @@ ERR_OUT_OF_MEMORY = 13
 ERR_ADMISSION_REFUSED = 14
 ERR_UNSUPPORTED = 15
 ERR_NO_DEVICE = 16
+ERR_BUSY = 17
 
 RESTORE_NONE, RESTORE_WIENER, RESTORE_SGRPROJ, RESTORE_SWITCHABLE = 0, 1, 2, 3
 MI_SIZE_SHIFT, MI_SKIP_BIT, MI_INTRA_BIT, MI_REF_SHIFT, MI_MODE_SHIFT, MI_SEG_SHIFT = 0, 5, 6, 8, 12, 24
@@ -155,12 +156,37 @@ class GenFrame(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", (C.c_uint16 * 2) * 5), ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", u32), ("max_batch", u32), ("arena_bytes", u64), ("largest", C.c_void_p)]
+
+
+class ServerDone(C.Structure):
+    _fields_ = [
+        ("ticket", u64), ("session", u32), ("slot", u32), ("status", C.c_int32), ("batch", u32),
+        ("arrival_us", C.c_double), ("launch_us", C.c_double), ("done_us", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("sessions_open", u32), ("lanes_busy", u32), ("frames_queued", u32), ("max_batch_seen", u32),
+        ("frames_submitted", u64), ("frames_done", u64), ("launches", u64), ("arena_bytes", u64),
+        ("arena_used", u64), ("lane_bytes", u64),
+    ]
+
+
 ABI_STRUCTS = {
     "oxav1_plane_i32": Plane, "oxav1_lf_ctx": LfCtx, "oxav1_lf_grids": LfGrids,
     "oxav1_cdef_params": CdefParams, "oxav1_cdef_ctx": CdefCtx, "oxav1_cdef_grids": CdefGrids,
     "oxav1_sr_ctx": SrCtx, "oxav1_lr_unit": LrUnit, "oxav1_lr_ctx": LrCtx,
     "oxav1_lr_grids": LrGrids, "oxav1_film_grain_params": FilmGrainParams,
     "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev,
+    "oxav1_tuning": Tuning, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
+    "oxav1_server_stats": ServerStats,
 }
 
 _P = C.POINTER
@@ -188,6 +214,21 @@ _SIGS = {
     "oxav1_pf_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "oxav1_pf_engine_stage_ms": (C.c_int, [C.c_void_p, C.c_char_p, _P(C.c_double), _P(u32)]),
     "oxav1_pf_engine_stream": (C.c_void_p, [C.c_void_p]),
+    "oxav1_pf_engine_query": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_engine_set_tuning": (C.c_int, [C.c_void_p, _P(Tuning)]),
+    "oxav1_pf_engine_get_tuning": (C.c_int, [C.c_void_p, _P(Tuning), _P(u32), _P(u32)]),
+    "oxav1_pf_engine_reserve": (C.c_int, [C.c_void_p, _P(FrameDesc), u32, _P(u64)]),
+    "oxav1_pf_server_now_us": (C.c_double, []),
+    "oxav1_pf_server_create": (C.c_int, [_P(C.c_void_p), _P(ServerConfig)]),
+    "oxav1_pf_server_destroy": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_server_open_session": (
+        C.c_int, [C.c_void_p, _P(FrameDesc), u32, u64, _P(u32), _P(C.c_void_p), _P(u64)]),
+    "oxav1_pf_server_close_session": (C.c_int, [C.c_void_p, u32]),
+    "oxav1_pf_server_slot": (C.c_int, [C.c_void_p, u32, u32, u32, C.c_int, _P(C.c_void_p), _P(u32)]),
+    "oxav1_pf_server_submit": (
+        C.c_int, [C.c_void_p, u32, _P(FrameDesc), _P(FrameDev), u32, C.c_double, _P(u64)]),
+    "oxav1_pf_server_poll": (C.c_int, [C.c_void_p, _P(ServerDone), u32, _P(u32)]),
+    "oxav1_pf_server_stats": (C.c_int, [C.c_void_p, _P(ServerStats)]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
     "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

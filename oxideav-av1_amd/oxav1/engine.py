@@ -5,6 +5,7 @@ one HIP runtime serves the whole process; every filter runs in the
 library's HIP kernels on the engine's stream.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -129,12 +130,24 @@ class DeviceFrame:
 class Engine:
     MODE_PLAIN, MODE_BATCH, MODE_GRAPH = 0, 1, 2
 
-    def __init__(self, max_frames=64):
+    def __init__(self, max_frames=64, tuning="table"):
+        """tuning: "table" applies the autotuner's saved table when there is
+        one, None keeps the built-in launch shapes, or an abi.Tuning."""
         self.lib = abi.load()
         h = C.c_void_p()
         abi.check(self.lib.oxav1_pf_engine_create(C.byref(h), None, max_frames), "engine_create")
         self.h = h
         self.frames = []
+        self.tuning_source = None
+        if tuning == "table":
+            from . import autotune
+            table = autotune.load_table()
+            if table is not None:
+                self.set_tuning(autotune.to_struct(table))
+                self.tuning_source = os.path.basename(autotune.TABLE)
+        elif tuning is not None:
+            self.set_tuning(tuning)
+            self.tuning_source = "given"
 
     def close(self):
         if self.h:
@@ -162,6 +175,26 @@ class Engine:
 
     def set_timing(self, on):
         abi.check(self.lib.oxav1_pf_engine_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def set_tuning(self, t):
+        abi.check(self.lib.oxav1_pf_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def tuning(self):
+        """(abi.Tuning, CU count, [workgroups of each stage that fit on a CU])."""
+        t, cus, fit = abi.Tuning(), C.c_uint32(), (C.c_uint32 * 5)()
+        abi.check(self.lib.oxav1_pf_engine_get_tuning(self.h, C.byref(t), C.byref(cus), fit), "get_tuning")
+        return t, cus.value, list(fit)
+
+    def stage_items(self, stage):
+        """Whether a stage has work in the prepared batch."""
+        for df in self.frames:
+            d = df.desc
+            on = {"lf": bool(d.apply_lf), "cdef": bool(d.apply_cdef),
+                  "sr": bool(d.use_superres) and d.upscaled_width > d.frame_width,
+                  "lr": bool(d.lr.uses_lr), "fg": bool(d.fg.apply_grain)}[stage]
+            if on:
+                return True
+        return False
 
     def stage_ms(self, stage):
         ms = C.c_double()

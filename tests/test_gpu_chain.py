@@ -94,3 +94,31 @@ def test_chain_4k_10bit_superres(oracle_mod):
     eng.sync()
     _check([df], oracle_mod, "cfg 2")
     eng.close()
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(blocks=1, sr_rows=4, lr_w=32),
+    dict(blocks=2, sr_rows=16, lr_w=64),
+    dict(blocks=0, sr_rows=32, lr_w=32),
+    dict(blocks=3, sr_rows=5, lr_w=0),
+])
+def test_chain_tuned_launch_shapes(knobs, oracle_mod):
+    """The autotuner's knobs (persistent workgroups per CU, superres rows
+    per item, loop-restoration tile width) leave every output byte as the
+    oracle's, for 8-bit and 10/12-bit frames in one batch."""
+    from oxav1 import abi
+    from oxav1.engine import Engine
+    frames = _frames([(gen_config(**kw), s) for kw, s in SMALL])
+    t = abi.Tuning()
+    for s in range(5):
+        for k in range(2):
+            t.blocks_per_cu[s][k] = knobs["blocks"]
+    for k in range(2):
+        t.sr_rows_per_item[k] = knobs["sr_rows"]
+        t.lr_tile_w[k] = knobs["lr_w"]
+    eng = Engine(max_frames=len(frames), tuning=t)
+    eng.prepare(frames)
+    eng.run(1)
+    eng.sync()
+    _check(frames, oracle_mod, f"tuning {knobs}")
+    eng.close()

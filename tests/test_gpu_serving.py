@@ -1,0 +1,109 @@
+"""GPU serving loop (oxav1_pf_server_*): decode sessions admitted by HBM
+footprint, frames batched across sessions onto lanes as they arrive, and
+every frame's bytes the same in the mix as filtered alone (and equal to the
+oracle's chain, frame_driver.rs:790-1116)."""
+import numpy as np
+import pytest
+
+from conftest import gen_config
+
+pytestmark = pytest.mark.gpu
+
+FG = dict(enable_fg=1, fg_num_y_points=6, fg_ar_lag=2, fg_chroma_points=3, fg_overlap=1)
+MIX = [
+    dict(),
+    dict(bit_depth=10, noise_amp=32, upscaled_width=160, frame_height=96, **FG),
+    dict(bit_depth=10, noise_amp=32, superres_denom=12, upscaled_width=192, frame_height=72),
+    dict(subsampling_x=0, subsampling_y=0, upscaled_width=120, frame_height=64, **FG),
+    dict(bit_depth=12, noise_amp=128, subsampling_y=0, upscaled_width=136, frame_height=88),
+    dict(mono=1, upscaled_width=104, frame_height=56),
+]
+
+
+def _sessions(cfgs, n_frames, fps, seed0=0x5E55):
+    from oxav1.serving import DecodeSession
+    return [DecodeSession(seed0 + i, n_frames=n_frames, fps=fps, pool=n_frames, slots=n_frames,
+                          gen_config=gen_config(**kw)) for i, kw in enumerate(cfgs)]
+
+
+def _assert_alone_is_oracle(sessions, expected, oracle_mod):
+    for (i, j), (ref, out) in expected.items():
+        ref_o, out_o, _ = oracle_mod.chain(sessions[i].frames[j])
+        for p in range(sessions[i].desc.num_planes):
+            assert np.array_equal(ref[p], ref_o[p]), f"session {i} frame {j} plane {p}: alone != oracle"
+            assert np.array_equal(out[p], out_o[p]), f"session {i} frame {j} plane {p}: alone grain != oracle"
+
+
+def test_serving_mix_equals_alone(oracle_mod):
+    """Six sessions of different shapes, depths and subsamplings, bursty
+    arrivals (lanes busy, so batches form): every frame is kept in its own
+    slot and compared with the same frame filtered alone."""
+    from oxav1.serving import Server, expected_outputs, run_serving
+    sessions = _sessions(MIX, n_frames=4, fps=3000.0)
+    expected = expected_outputs(sessions)
+    _assert_alone_is_oracle(sessions, expected, oracle_mod)
+    srv = Server(lanes=3, max_batch=6)
+    try:
+        res = run_serving(srv, sessions, expected, max_seconds=60)
+    finally:
+        srv.close()
+    print("serving mix:", {k: v for k, v in res.items() if k != "stats"}, res["stats"])
+    assert res["frames"] == 4 * len(MIX)
+    assert res["verified_frames"] == 4 * len(MIX)
+    assert res["mismatches"] == []
+    assert res["stats"]["frames_done"] == res["stats"]["frames_submitted"] == 4 * len(MIX)
+
+
+def test_serving_admission_by_footprint():
+    """An arena that holds two sessions: the others wait and are admitted
+    as sessions finish and free their slots; results still match alone."""
+    from oxav1.serving import Server, expected_outputs, run_serving
+    sessions = _sessions([MIX[1]] * 5, n_frames=3, fps=2000.0, seed0=0xAD17)
+    expected = expected_outputs(sessions)
+    probe = Server(lanes=1, max_batch=1, arena_bytes=1 << 26)
+    try:
+        sid, _, fp = probe.open_session(sessions[0].desc, sessions[0].slots, sessions[0].input_bytes)
+        assert probe.close_session(sid)
+    finally:
+        probe.close()
+    srv = Server(lanes=2, max_batch=4, arena_bytes=int(fp * 2.5))
+    try:
+        a = srv.open_session(sessions[0].desc, 3, sessions[0].input_bytes)
+        b = srv.open_session(sessions[0].desc, 3, sessions[0].input_bytes)
+        assert a is not None and b is not None
+        assert srv.open_session(sessions[0].desc, 3, sessions[0].input_bytes) is None  # refused
+        assert srv.close_session(a[0]) and srv.close_session(b[0])
+        assert srv.stats()["arena_used"] == 0
+        res = run_serving(srv, sessions, expected, max_seconds=60)
+    finally:
+        srv.close()
+    assert res["peak_sessions_open"] == 2
+    assert res["frames"] == 15 and res["verified_frames"] == 15
+    assert res["mismatches"] == []
+
+
+def test_serving_slot_busy_until_done():
+    """A slot with a frame queued or in flight refuses another frame, and
+    its session cannot close until the frame is done."""
+    from oxav1.serving import Server
+    s = _sessions([MIX[0]], n_frames=1, fps=30.0)[0]
+    srv = Server(lanes=1, max_batch=1)
+    try:
+        assert s.admit(srv)
+        desc, dev, slot = s.frame_job(0)
+        assert srv.submit(s.sid, desc, dev, slot, srv.now_us()) is not None
+        assert srv.submit(s.sid, desc, dev, slot, srv.now_us()) is None
+        assert not srv.close_session(s.sid)
+        done = []
+        t0 = srv.now_us()
+        while not done:
+            done = srv.poll()
+            assert srv.now_us() - t0 < 30e6, "frame did not finish"
+        assert len(done) == 1 and done[0].status == 0 and done[0].batch == 1
+        assert srv.submit(s.sid, desc, dev, slot, srv.now_us()) is not None
+        while srv.stats()["frames_done"] < 2:
+            srv.poll()
+            assert srv.now_us() - t0 < 30e6, "frame did not finish"
+        assert srv.close_session(s.sid)
+    finally:
+        srv.close()
