@@ -181,7 +181,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
-              "(frames and sessions would shard across GPUs with no exchange; see DESIGN.md)",
+              "(frames and sessions shard across GPUs with no exchange between them)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -15,8 +15,11 @@
  *     sequentially vs. all-edges-parallel, CDEF direction search vs. a
  *     brute-force numpy argmax, superres vs. a numpy polyphase convolution,
  *     LR box sums vs. numpy window sums, film-grain LFSR vs. an
- *     independent bit-serial model.
- * Functions without such a pin are marked "parity unpinned" in DESIGN.md.
+ *     independent bit-serial model;
+ *   - numpy models written from the spec equations for the Wiener and
+ *     self-guided filters on non-uniform input, the chroma grain
+ *     recurrence with its luma tap, and the noise stripes / overlap
+ *     blending / scaled blend of film-grain synthesis.
  */
 #define _POSIX_C_SOURCE 199309L
 #include "oxav1_oracle.h"

@@ -175,7 +175,7 @@ __device__ __forceinline__ CdefTaps cdef_taps(const CdefBlk &b, bool chroma, int
 // column in steps of blockDim.x >> row_log2 rows.
 // CHECK: per-tap "inside the plane" test (units touching the frame edge).
 template <typename L, typename T, bool CHECK>
-__device__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
+__device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
                                   int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
                                   int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
     const int xx = threadIdx.x & ((1 << row_log2) - 1);
@@ -231,7 +231,7 @@ __device__ __forceinline__ int div_magic(int i, uint32_t m) {
 }
 
 template <typename L, typename T>
-__device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
+__device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
                           const G<T> *__restrict__ src, uint32_t spitch) {
     const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
     const uint32_t m = ((1u << 20) + (uint32_t)ls - 1) / (uint32_t)ls;
@@ -248,7 +248,7 @@ __device__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t 
 
 // One 64x64 unit (ur, uc in 64-luma units): deb -> cdf for all planes.
 template <typename T>
-__device__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
+__device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
                           CdefBlk *blk, CdefTaps *taps, int32_t *costs /* 64 x 8 */) {
     using L = typename LdsOf<T>::type;
     const int tid = threadIdx.x;

@@ -67,7 +67,7 @@ struct LfShared {
 };
 
 template <typename T>
-__device__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *region, LfShared &sh) {
+__device__ __forceinline__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *region, LfShared &sh) {
     const uint32_t p = plane_of(J.item_base[kStageLf], 3, local);
     const uint32_t t = local - J.item_base[kStageLf][p];
     const uint32_t ty = t / J.lf_tiles_x[p], tx = t - ty * J.lf_tiles_x[p];
@@ -76,7 +76,7 @@ __device__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::ty
 }
 
 template <typename T>
-__device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
+__device__ __forceinline__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
                           CdefTaps *taps, int32_t *costs) {
     const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
     cdef_unit<T>(J, ur, uc, tile, blk, taps, costs);
@@ -87,7 +87,7 @@ __device__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::
 // four rows around each stripe boundary k >= 1: rows k*(64>>sy) - (8>>sy) - 2
 // .. +1 (the rows loop restoration reads from it).
 template <typename T>
-__device__ void sr_item(const FrameJob &J, uint32_t local) {
+__device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local) {
     const uint32_t p6 = plane_of(J.item_base[kStageSr], 6, local);
     const uint32_t chunk = local - J.item_base[kStageSr][p6];
     const uint32_t p = p6 % 3;
@@ -110,7 +110,7 @@ __device__ void sr_item(const FrameJob &J, uint32_t local) {
 }
 
 template <typename T>
-__device__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
+__device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
     const uint32_t p = plane_of(J.item_base[kStageLr], 3, local);
     const uint32_t t = local - J.item_base[kStageLr][p];
     const uint32_t per_stripe = J.lr_tiles_x[p] * lr_chunks(p ? J.sy : 0);
@@ -120,7 +120,7 @@ __device__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf
 }
 
 template <typename T>
-__device__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
+__device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
     const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
     const uint32_t t = local - J.item_base[kStageFg][p];
     const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
@@ -154,7 +154,7 @@ __device__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
     }
 
 template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_lf(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+__device__ __forceinline__ void lf_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L region[kLfRegion * kLfStride];
@@ -172,8 +172,8 @@ __global__ __launch_bounds__(kThreads) void k_lf(const FrameJob *jobs, const uin
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                   uint32_t total, uint32_t *counter) {
+__device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+                                                 uint32_t total, uint32_t *counter) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L tile[kCdefLw * kCdefLw];
     __shared__ CdefBlk blk[64];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_cdef(const FrameJob *jobs, const u
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_sr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+__device__ __forceinline__ void sr_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_sr(const FrameJob *jobs, const uin
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_lr(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+__device__ __forceinline__ void lr_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ LrSmem<L> S;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_lr(const FrameJob *jobs, const uin
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_fg(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
+__device__ __forceinline__ void fg_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter) {
     __shared__ uint8_t lut[768];
     OXAV1_ITEM_LOOP({
@@ -238,6 +238,28 @@ __global__ __launch_bounds__(kThreads) void k_fg(const FrameJob *jobs, const uin
         }
     })
 }
+
+
+// Each stage has a kernel per sample path (KIND) with the compiler's register
+// budget, and engine variants k_<stage>_w<W> whose budget allows at least W
+// waves per SIMD; the tuning picks one per stage and sample size.
+#define OXAV1_STAGE_KERNELS(NAME)                                                                         \
+    template <int KIND>                                                                                   \
+    __global__ __launch_bounds__(kThreads) void k_##NAME(const FrameJob *jobs, const uint32_t *prefix,   \
+                                                         uint32_t njobs, uint32_t total, uint32_t *counter) { \
+        NAME##_body<KIND>(jobs, prefix, njobs, total, counter);                                          \
+    }                                                                                                     \
+    template <int W>                                                                                      \
+    __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W))) void k_##NAME##_w(    \
+        const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t total, uint32_t *counter) { \
+        NAME##_body<0>(jobs, prefix, njobs, total, counter);                                             \
+    }
+OXAV1_STAGE_KERNELS(lf)
+OXAV1_STAGE_KERNELS(cdef)
+OXAV1_STAGE_KERNELS(sr)
+OXAV1_STAGE_KERNELS(lr)
+OXAV1_STAGE_KERNELS(fg)
+#undef OXAV1_STAGE_KERNELS
 
 // One workgroup per frame; frames without film grain return at once.
 __global__ __launch_bounds__(kThreads) void k_fg_prep(const FrameJob *jobs, uint32_t njobs) {
@@ -358,6 +380,20 @@ static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw, uint32_t pref =
 
 static uint32_t lr_stripe_count(uint32_t pey, uint32_t sy) { return ((pey << sy) + 8) / 64 + 1; }
 
+using StageKernel = void (*)(const FrameJob *, const uint32_t *, uint32_t, uint32_t, uint32_t *);
+constexpr int kWaveVariants = 4;
+constexpr int kWaveOf[kWaveVariants] = {0, 6, 7, 8};  // 0 = the compiler's choice
+#define OXAV1_VARIANTS(K) {K<0>, K##_w<6>, K##_w<7>, K##_w<8>}
+static StageKernel kEngineKernels[5][kWaveVariants] = {OXAV1_VARIANTS(k_lf), OXAV1_VARIANTS(k_cdef),
+                                                       OXAV1_VARIANTS(k_sr), OXAV1_VARIANTS(k_lr),
+                                                       OXAV1_VARIANTS(k_fg)};
+#undef OXAV1_VARIANTS
+static int wave_variant(uint32_t waves) {
+    for (int w = 1; w < kWaveVariants; w++)
+        if ((uint32_t)kWaveOf[w] == waves) return w;
+    return 0;
+}
+
 }  // namespace oxav1
 
 using namespace oxav1;
@@ -401,7 +437,7 @@ struct oxav1_pf_engine {
     hipEvent_t ev[8] = {};
     double stage_sum[6] = {0, 0, 0, 0, 0, 0};
     uint32_t stage_cnt[6] = {0, 0, 0, 0, 0, 0};
-    int persistent_blocks[5] = {0, 0, 0, 0, 0};  // workgroups per CU that fit, per stage
+    int persistent_blocks[5][kWaveVariants] = {};  // workgroups per CU that fit, per stage and variant
     oxav1_tuning tuning = {};                       // zero = defaults
     int batch_kind = 1;                             // 0 = mostly 8-bit, 1 = mostly 10/12-bit
 };
@@ -638,15 +674,15 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
     if (e->num_cus <= 0) e->num_cus = 256;
     // persistent grid per stage: every workgroup slot the kernel can occupy
     {
-        const void *ks[5] = {(const void *)k_lf<0>, (const void *)k_cdef<0>, (const void *)k_sr<0>,
-                             (const void *)k_lr<0>, (const void *)k_fg<0>};
-        for (int s = 0; s < 5; s++) {
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[s], kThreads, 0) != hipSuccess ||
-                per_cu <= 0)
-                per_cu = 2;
-            e->persistent_blocks[s] = per_cu;
-        }
+        for (int s = 0; s < 5; s++)
+            for (int w = 0; w < kWaveVariants; w++) {
+                int per_cu = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kEngineKernels[s][w], kThreads,
+                                                                 0) != hipSuccess ||
+                    per_cu <= 0)
+                    per_cu = 2;
+                e->persistent_blocks[s][w] = per_cu;
+            }
     }
     e->max_frames = max_frames ? max_frames : 1;
     if (hip_stream) {
@@ -742,8 +778,6 @@ extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_des
     return OXAV1_OK;
 }
 
-using StageKernel = void (*)(const FrameJob *, const uint32_t *, uint32_t, uint32_t, uint32_t *);
-static StageKernel kEngineKernels[5] = {k_lf<0>, k_cdef<0>, k_sr<0>, k_lr<0>, k_fg<0>};
 
 // Plain path: per frame, one launch per stage sized to the frame.
 static int enqueue_plain(oxav1_pf_engine *e, bool timed) {
@@ -762,7 +796,8 @@ static int enqueue_plain(oxav1_pf_engine *e, bool timed) {
             if (s == kStageFg && J.apply_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
             const uint32_t items = stage_items(J, s);
             if (items) {
-                hipLaunchKernelGGL(kEngineKernels[s], dim3(items), dim3(kThreads), 0, e->stream, jd + f,
+                const int w = wave_variant(e->tuning.waves_per_eu[s][J.kind == kU8 ? 0 : 1]);
+                hipLaunchKernelGGL(kEngineKernels[s][w], dim3(items), dim3(kThreads), 0, e->stream, jd + f,
                                    (const uint32_t *)nullptr, 1u, items, (uint32_t *)nullptr);
                 HIP_TRY(hipGetLastError());
             }
@@ -793,11 +828,12 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
         const uint32_t total = e->totals[s];
         if (s == kStageFg && e->any_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
         if (total) {
+            const int w = wave_variant(e->tuning.waves_per_eu[s][e->batch_kind]);
             const uint32_t per_cu = e->tuning.blocks_per_cu[s][e->batch_kind];
-            const uint32_t fit = (uint32_t)e->persistent_blocks[s];
+            const uint32_t fit = (uint32_t)e->persistent_blocks[s][w];
             const uint32_t grid =
                 std::min<uint32_t>(total, (uint32_t)e->num_cus * (per_cu ? std::min(per_cu, fit) : fit));
-            hipLaunchKernelGGL(kEngineKernels[s], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
+            hipLaunchKernelGGL(kEngineKernels[s][w], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
                                e->n, total, ctr + s);
             HIP_TRY(hipGetLastError());
         }
@@ -934,6 +970,8 @@ extern "C" int oxav1_pf_engine_reserve(oxav1_pf_engine *e, const oxav1_frame_des
 extern "C" int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning *t) {
     if (!e || !t) return OXAV1_ERR_INVALID_ARG;
     for (int k = 0; k < 2; k++) {
+        for (int st = 0; st < 5; st++)
+            if (t->waves_per_eu[st][k] && !wave_variant(t->waves_per_eu[st][k])) return OXAV1_ERR_INVALID_ARG;
         if (t->sr_rows_per_item[k] > 64) return OXAV1_ERR_INVALID_ARG;
         if (t->lr_tile_w[k] != 0 && t->lr_tile_w[k] != 32 && t->lr_tile_w[k] != 64) return OXAV1_ERR_INVALID_ARG;
     }
@@ -942,12 +980,13 @@ extern "C" int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning
 }
 
 extern "C" int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, uint32_t *num_cus,
-                                          uint32_t fit_blocks_per_cu[5]) {
+                                          uint32_t fit_blocks_per_cu[5][4]) {
     if (!e) return OXAV1_ERR_INVALID_ARG;
     if (t) *t = e->tuning;
     if (num_cus) *num_cus = (uint32_t)e->num_cus;
     if (fit_blocks_per_cu)
-        for (int s = 0; s < 5; s++) fit_blocks_per_cu[s] = (uint32_t)e->persistent_blocks[s];
+        for (int s = 0; s < 5; s++)
+            for (int w = 0; w < kWaveVariants; w++) fit_blocks_per_cu[s][w] = (uint32_t)e->persistent_blocks[s][w];
     return OXAV1_OK;
 }
 

@@ -333,7 +333,7 @@ __device__ __forceinline__ int32_t fg_scale(const uint8_t *t, int32_t index, uin
 // template row pointers are resolved once per lane, and the overlap blends
 // only touch the first columns of a block and the first rows of a stripe.
 template <typename T>
-__device__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
+__device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
                               const uint8_t *lut /* LDS [3][256] */) {
     const G<FgScratch> *S = (const G<FgScratch> *)J.fgs;
     const oxav1_film_grain_params &fg = J.fg;

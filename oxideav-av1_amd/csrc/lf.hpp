@@ -71,7 +71,7 @@ __device__ __forceinline__ uint32_t lf_level_at(const FrameJob &J, uint32_t row,
 // "no filtering", else (filter_size | lvl << 8).  (by, bx) index the plane's
 // 4x4 blocks, i.e. the loop cursor row = by << sy, col = bx << sx
 // (loop_filter.rs:282-292).
-__device__ uint32_t lf_decide(const FrameJob &J, uint32_t plane, uint32_t pass, int32_t by,
+__device__ __forceinline__ uint32_t lf_decide(const FrameJob &J, uint32_t plane, uint32_t pass, int32_t by,
                               int32_t bx) {
     if (by < 0 || bx < 0) return 0;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void lf_wide(const Reg &R, const int32_t *v, int32_t 
 // sample_filtering + filter_mask + narrow/wide filter (loop_filter.rs:582-822)
 // for one sample position of one edge.
 template <typename L, bool EDGE>
-__device__ void lf_filter_sample(const LfRegion<L, EDGE> &R, int32_t x, int32_t y, int32_t dx, int32_t dy,
+__device__ __forceinline__ void lf_filter_sample(const LfRegion<L, EDGE> &R, int32_t x, int32_t y, int32_t dx, int32_t dy,
                                  uint32_t plane, uint32_t fs, uint32_t lvl, uint32_t sharp, uint32_t bd) {
     // strength (loop_filter.rs:502-523)
     const int32_t shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
@@ -253,7 +253,7 @@ __device__ void lf_filter_sample(const LfRegion<L, EDGE> &R, int32_t x, int32_t 
 
 // Passes 0 and 1 over the loaded region (see lf_tile).
 template <typename L, bool EDGE>
-__device__ void lf_passes(const LfRegion<L, EDGE> &R, int32_t X0, int32_t Y0, uint32_t plane,
+__device__ __forceinline__ void lf_passes(const LfRegion<L, EDGE> &R, int32_t X0, int32_t Y0, uint32_t plane,
                           const uint16_t *dec0, const uint16_t *dec1, uint32_t sharp, uint32_t bd) {
     const int tid = threadIdx.x, nt = blockDim.x;
     // pass 0: vertical edges x every loaded row; consecutive lanes take
@@ -281,7 +281,7 @@ __device__ void lf_passes(const LfRegion<L, EDGE> &R, int32_t X0, int32_t Y0, ui
 
 // One 64x64 tile of one plane: src (T) -> dst (T).
 template <typename T>
-__device__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t tx,
+__device__ __forceinline__ void lf_tile(const FrameJob &J, uint32_t plane, uint32_t ty, uint32_t tx,
                         const G<T> *__restrict__ src, uint32_t spitch, G<T> *__restrict__ dst,
                         uint32_t dpitch, typename LdsOf<T>::type *sm, uint16_t *dec0, uint16_t *dec1) {
     using L = typename LdsOf<T>::type;

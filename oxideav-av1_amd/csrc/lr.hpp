@@ -94,7 +94,7 @@ __device__ __forceinline__ void lr_sgr_consts(int32_t r, int32_t e, int32_t *n, 
 // One tile: (plane, stripe, column tile) -> out (cropped frame coordinates
 // equal plane coordinates).  Writes rows [ys, ye] x cols [xs, xe].
 template <typename T>
-__device__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
+__device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
                         LrSmem<typename LdsOf<T>::type> &S, G<T> *__restrict__ out, uint32_t opitch) {
     using L = typename LdsOf<T>::type;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;

@@ -28,7 +28,7 @@ __constant__ int16_t c_upscale_filter[64][8];  // filled from kUpscaleFilter at 
 // -(1<<14) + initialSubpelX + x*stepX, integer part px, phase from bits 8..13,
 // taps at Clip3(0, maxX, px+k-3) further clamped to the stored row.
 template <typename T>
-__device__ void sr_rows(const G<T> *__restrict__ in, uint32_t ipitch, int32_t in_cols, G<T> *__restrict__ out,
+__device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ipitch, int32_t in_cols, G<T> *__restrict__ out,
                         uint32_t opitch, int32_t out_cols, int32_t y0, int32_t nrows, int32_t ylim, int32_t step,
                         int32_t init, int32_t maxx, int32_t bd) {
     const int32_t maxv = (1 << bd) - 1;

@@ -157,7 +157,8 @@ class GenFrame(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("blocks_per_cu", (C.c_uint16 * 2) * 5), ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2)]
+    _fields_ = [("blocks_per_cu", (C.c_uint16 * 2) * 5), ("waves_per_eu", (u8 * 2) * 5),
+                ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2)]
 
 
 class ServerConfig(C.Structure):
@@ -216,7 +217,7 @@ _SIGS = {
     "oxav1_pf_engine_stream": (C.c_void_p, [C.c_void_p]),
     "oxav1_pf_engine_query": (C.c_int, [C.c_void_p]),
     "oxav1_pf_engine_set_tuning": (C.c_int, [C.c_void_p, _P(Tuning)]),
-    "oxav1_pf_engine_get_tuning": (C.c_int, [C.c_void_p, _P(Tuning), _P(u32), _P(u32)]),
+    "oxav1_pf_engine_get_tuning": (C.c_int, [C.c_void_p, _P(Tuning), _P(u32), _P((u32 * 4) * 5)]),
     "oxav1_pf_engine_reserve": (C.c_int, [C.c_void_p, _P(FrameDesc), u32, _P(u64)]),
     "oxav1_pf_server_now_us": (C.c_double, []),
     "oxav1_pf_server_create": (C.c_int, [_P(C.c_void_p), _P(ServerConfig)]),

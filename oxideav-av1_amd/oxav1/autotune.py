@@ -3,10 +3,13 @@
 For each filter stage and each sample size (8-bit frames, 10/12-bit frames)
 it measures the stage's kernel time (HIP events on the engine's stream,
 batched persistent path) over a sweep of
+  - the kernel's register budget: the compiled variant that allows at least
+    6, 7 or 8 waves per SIMD, or the compiler's own choice,
   - persistent workgroups per CU (1 .. as many as fit),
   - superres rows per work item (4, 8, 16, 32),
   - loop-restoration tile width (32, 64),
-one knob at a time, keeps the fastest, and checks that the outputs of the
+one knob at a time, keeps the fastest (a candidate must beat the current
+choice by more than 1.5% to replace it), and checks that the outputs of the
 chosen shape are byte-identical to those of the default shape (no knob
 changes arithmetic).  The result is a small JSON table per device
 (tuning_<arch>.json beside this file) that Engine applies when it exists.
@@ -32,6 +35,7 @@ def to_struct(table):
     for s, name in enumerate(STAGES):
         for k, kind in enumerate(KINDS):
             t.blocks_per_cu[s][k] = int(table.get("blocks_per_cu", {}).get(kind, {}).get(name, 0))
+            t.waves_per_eu[s][k] = int(table.get("waves_per_eu", {}).get(kind, {}).get(name, 0))
     for k, kind in enumerate(KINDS):
         t.sr_rows_per_item[k] = int(table.get("sr_rows_per_item", {}).get(kind, 0))
         t.lr_tile_w[k] = int(table.get("lr_tile_w", {}).get(kind, 0))
@@ -68,9 +72,20 @@ def _same(a, b):
     return all(np.array_equal(x, y) for fa, fb in zip(a, b) for x, y in zip(fa[0] + fa[1], fb[0] + fb[1]))
 
 
-def tune_kind(dfs, kind, steps=4, log=print):
+MARGIN = 0.985
+WAVES = (0, 6, 7, 8)
+
+
+def _pick(res, current):
+    """The fastest candidate, unless it beats `current` by less than the margin."""
+    best = min(res, key=res.get)
+    return best if res[best] < res[current] * MARGIN else current
+
+
+def tune_kind(dfs, kind, steps=6, log=print):
     """Sweep the knobs for one sample size on these device frames; returns
-    (blocks_per_cu {stage: n}, sr_rows, lr_tile_w, timings)."""
+    (waves_per_eu {stage: w}, blocks_per_cu {stage: n}, sr_rows, lr_tile_w,
+    timings)."""
     from .engine import Engine
     k = KINDS.index(kind)
     eng = Engine(max_frames=len(dfs), tuning=None)
@@ -84,43 +99,53 @@ def tune_kind(dfs, kind, steps=4, log=print):
         _, cus, fit = eng.tuning()
         active = [s for s in STAGES if eng.stage_items(s)]
         timings = {}
-        best_bpc = {}
+        best_w, best_bpc = {}, {}
+
+        def measure(stage):
+            eng.set_tuning(t)
+            eng.prepare(dfs)
+            return _stage_ms(eng, stage, steps)
+
         for si, s in enumerate(STAGES):
             if s not in active:
                 continue
-            cands = sorted({c for c in (1, 2, 3, 4, 6, 8, fit[si]) if 1 <= c <= fit[si]})
             res = {}
-            for c in cands:
+            for w in WAVES:
+                t.waves_per_eu[si][k] = w
+                t.blocks_per_cu[si][k] = 0
+                res[w] = measure(s)
+            w = _pick(res, 0)
+            t.waves_per_eu[si][k] = w
+            best_w[s] = w
+            timings[f"{s}_waves_per_eu"] = {str(c): round(v, 4) for c, v in res.items()}
+            log(f"  {kind} {s}: waves/SIMD variant {res} -> {w}")
+            nfit = fit[si][WAVES.index(w)]
+            res = {0: res[w]}
+            for c in sorted({c for c in (1, 2, 3, 4, 6, 8) if c < nfit}):
                 t.blocks_per_cu[si][k] = c
-                eng.set_tuning(t)
-                eng.prepare(dfs)
-                res[c] = _stage_ms(eng, s, steps)
-            best = min(res, key=res.get)
-            t.blocks_per_cu[si][k] = best
-            best_bpc[s] = best
+                res[c] = measure(s)
+            c = _pick(res, 0)
+            t.blocks_per_cu[si][k] = c
+            best_bpc[s] = c
             timings[f"{s}_blocks_per_cu"] = {str(c): round(v, 4) for c, v in res.items()}
-            log(f"  {kind} {s}: blocks/CU {res} -> {best}")
+            log(f"  {kind} {s}: blocks/CU (0 = all {nfit} that fit) {res} -> {c}")
         sr_best = 0
         if "sr" in active:
             res = {}
-            for r in (4, 8, 16, 32):
+            for r in (8, 4, 16, 32):
                 t.sr_rows_per_item[k] = r
-                eng.set_tuning(t)
-                eng.prepare(dfs)
-                res[r] = _stage_ms(eng, "sr", steps)
-            sr_best = min(res, key=res.get)
+                res[r] = measure("sr")
+            sr_best = _pick(res, 8)
             t.sr_rows_per_item[k] = sr_best
             timings["sr_rows_per_item"] = {str(c): round(v, 4) for c, v in res.items()}
             log(f"  {kind} sr rows/item {res} -> {sr_best}")
         lr_best = 0
         if "lr" in active:
             res = {}
-            for w in (32, 64):
+            for w in (64, 32):
                 t.lr_tile_w[k] = w
-                eng.set_tuning(t)
-                eng.prepare(dfs)
-                res[w] = _stage_ms(eng, "lr", steps)
-            lr_best = min(res, key=res.get)
+                res[w] = measure("lr")
+            lr_best = _pick(res, 64)
             t.lr_tile_w[k] = lr_best
             timings["lr_tile_w"] = {str(c): round(v, 4) for c, v in res.items()}
             log(f"  {kind} lr tile width {res} -> {lr_best}")
@@ -128,7 +153,7 @@ def tune_kind(dfs, kind, steps=4, log=print):
         eng.prepare(dfs)
         if not _same(_outputs(eng, dfs), ref):
             raise RuntimeError(f"{kind}: tuned launch shape changed the output")
-        return best_bpc, sr_best, lr_best, timings
+        return best_w, best_bpc, sr_best, lr_best, timings
     finally:
         eng.close()
 
@@ -157,16 +182,17 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--write", action="store_true", help=f"save the table to {TABLE}")
     ap.add_argument("--out", default=None, help="save the table here instead")
-    ap.add_argument("--frames", type=int, default=4)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--frames", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=6)
     args = ap.parse_args()
     abi.check(abi.load().oxav1_pf_init(), "init")
     from .engine import device_info
     t0 = time.time()
-    table = {"device": {"cus": device_info()[0]}, "blocks_per_cu": {}, "sr_rows_per_item": {},
-             "lr_tile_w": {}, "timings_ms": {}}
+    table = {"device": {"cus": device_info()[0]}, "waves_per_eu": {}, "blocks_per_cu": {},
+             "sr_rows_per_item": {}, "lr_tile_w": {}, "timings_ms": {}}
     for kind, dfs in workloads(args.frames).items():
-        bpc, sr, lr, tm = tune_kind(dfs, kind, args.steps)
+        wv, bpc, sr, lr, tm = tune_kind(dfs, kind, args.steps)
+        table["waves_per_eu"][kind] = wv
         table["blocks_per_cu"][kind] = bpc
         table["sr_rows_per_item"][kind] = sr
         table["lr_tile_w"][kind] = lr

@@ -180,10 +180,12 @@ class Engine:
         abi.check(self.lib.oxav1_pf_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tuning(self):
-        """(abi.Tuning, CU count, [workgroups of each stage that fit on a CU])."""
-        t, cus, fit = abi.Tuning(), C.c_uint32(), (C.c_uint32 * 5)()
+        """(abi.Tuning, CU count, fit) where fit[stage][v] is how many
+        workgroups of the stage's register-budget variant v (waves per SIMD
+        0 = compiler's choice, 6, 7, 8) fit on one CU."""
+        t, cus, fit = abi.Tuning(), C.c_uint32(), ((C.c_uint32 * 4) * 5)()
         abi.check(self.lib.oxav1_pf_engine_get_tuning(self.h, C.byref(t), C.byref(cus), fit), "get_tuning")
-        return t, cus.value, list(fit)
+        return t, cus.value, [list(r) for r in fit]
 
     def stage_items(self, stage):
         """Whether a stage has work in the prepared batch."""

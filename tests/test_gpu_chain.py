@@ -97,15 +97,17 @@ def test_chain_4k_10bit_superres(oracle_mod):
 
 
 @pytest.mark.parametrize("knobs", [
-    dict(blocks=1, sr_rows=4, lr_w=32),
-    dict(blocks=2, sr_rows=16, lr_w=64),
-    dict(blocks=0, sr_rows=32, lr_w=32),
-    dict(blocks=3, sr_rows=5, lr_w=0),
+    dict(blocks=1, sr_rows=4, lr_w=32, waves=0),
+    dict(blocks=2, sr_rows=16, lr_w=64, waves=6),
+    dict(blocks=0, sr_rows=32, lr_w=32, waves=7),
+    dict(blocks=3, sr_rows=5, lr_w=0, waves=8),
+    dict(blocks=0, sr_rows=0, lr_w=0, waves=8),
 ])
 def test_chain_tuned_launch_shapes(knobs, oracle_mod):
-    """The autotuner's knobs (persistent workgroups per CU, superres rows
-    per item, loop-restoration tile width) leave every output byte as the
-    oracle's, for 8-bit and 10/12-bit frames in one batch."""
+    """The autotuner's knobs (register-budget variant, persistent workgroups
+    per CU, superres rows per item, loop-restoration tile width) leave every
+    output byte as the oracle's, for 8-bit and 10/12-bit frames in one
+    batch, on the batched and the plain path."""
     from oxav1 import abi
     from oxav1.engine import Engine
     frames = _frames([(gen_config(**kw), s) for kw, s in SMALL])
@@ -113,6 +115,7 @@ def test_chain_tuned_launch_shapes(knobs, oracle_mod):
     for s in range(5):
         for k in range(2):
             t.blocks_per_cu[s][k] = knobs["blocks"]
+            t.waves_per_eu[s][k] = knobs["waves"]
     for k in range(2):
         t.sr_rows_per_item[k] = knobs["sr_rows"]
         t.lr_tile_w[k] = knobs["lr_w"]
@@ -121,4 +124,10 @@ def test_chain_tuned_launch_shapes(knobs, oracle_mod):
     eng.run(1)
     eng.sync()
     _check(frames, oracle_mod, f"tuning {knobs}")
+    for df in frames:
+        df.clear_outputs()
+        eng.prepare([df])
+        eng.run(0)
+        eng.sync()
+    _check(frames, oracle_mod, f"tuning {knobs}, plain")
     eng.close()
