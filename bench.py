@@ -52,6 +52,44 @@ def stages_on(frames):
     return "+".join(names[k] for k in stage_keys(frames))
 
 
+def host_info():
+    """Host CPU facts for the CPU baseline: model, logical CPUs, usable cores."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def dropin_rate(frame):
+    """The reference's interface end to end: the frame driver's post-filter
+    sequence through the five drop-in calls on host int32 planes (each call
+    stages its planes to HBM and back over PCIe).  Returns a dict."""
+    import numpy as np
+
+    from oxav1 import dropin
+    calls = dropin.gpu()
+    dropin.frame_driver_chain(calls, frame)  # warm
+    reps, stage = 3, []
+    t = time.perf_counter()
+    for _ in range(reps):
+        st = []
+        dropin.frame_driver_chain(calls, frame, st)
+        stage.append(st)
+    s = (time.perf_counter() - t) / reps
+    d = frame.desc
+    return {"mpx_s": round(d.upscaled_width * d.frame_height / s / 1e6, 2), "ms_per_frame": round(s * 1e3, 2),
+            "stage_ms": {k: round(v * 1e3, 2) for k, v in zip(("lf", "cdef", "sr", "lr", "fg"),
+                                                              np.mean(stage, axis=0))},
+            "what": f"one {d.upscaled_width}x{d.frame_height} frame through the five drop-in calls "
+                    f"(int32 host planes, PCIe-inclusive), the reference's own interface"}
+
+
 def stage_keys(frames):
     """Engine stage names that do work on at least one of the frames."""
     keys = ["lf", "cdef"]
@@ -165,6 +203,9 @@ def main():
     ap.add_argument("--cfg", type=int, default=2, help="workload configuration (2 = 4K 10-bit)")
     ap.add_argument("--cpu-frames", type=int, default=4, help="frames timed on the CPU oracle (about 10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-parallel", type=int, default=1,
+                    help="also time the CPU oracle frame-parallel over the host cores (1/0)")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (PCIe-inclusive) timing")
     ap.add_argument("--profile", action="store_true", help="timed batched steps only (for rocprofv3)")
     ap.add_argument("--serve", action="store_true",
                     help="serving loop instead: concurrent decode sessions (720p/1080p/4K, 8/10-bit 4:2:0), "
@@ -309,10 +350,28 @@ def main():
             ref_g, out_g = plain[i]
             oracle_exact += all(np.array_equal(a, b) for a, b in zip(ref_o + out_o, ref_g + out_g))
         which = f"the first {n}" if pick == list(range(n)) else f"{n} evenly spaced"
+        stage_ns = np.mean([o[2] for o in outs], axis=0)
         cpu = {"value": round(sum(fpx[i] for i in pick) / cpu_s / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
                "kind": "port",
                "sample": f"{which} of the {len(frames)} frames of the step ({describe([frames[i] for i in pick])}), "
-                         f"full chain through the C oracle, single thread, {cpu_s:.1f} s"}
+                         f"full chain through the C oracle, single thread, {cpu_s:.1f} s",
+               "stage_ms_per_frame": {k: round(v * 1e-6, 1) for k, v in zip(("lf", "cdef", "sr", "lr", "fg"),
+                                                                          stage_ns)},
+               "host": host_info()}
+        if args.cpu_parallel:
+            # frame-parallel: one frame per thread (the oracle is reentrant and
+            # ctypes releases the GIL for the call)
+            from concurrent.futures import ThreadPoolExecutor
+            threads = min(len(frames), cpu["host"]["usable_cores"], 16)
+            t = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(oracle.chain, frames))
+            par_s = time.perf_counter() - t
+            cpu["frame_parallel"] = {"value": round(total_px / par_s / 1e6, 4), "cores": threads,
+                                     "sample": f"all {len(frames)} frames of the step, one frame per "
+                                               f"thread, {par_s:.1f} s"}
+
+    dropin = None if args.no_dropin else dropin_rate(frames[0])
 
     value = total_px / s_batch / 1e6
     line = {
@@ -358,6 +417,7 @@ def main():
         "byte_exact": {"batched_eq_plain_frames": exact_paths, "frames": len(frames),
                        "oracle_checked": len(pick) if cpu else 0,
                        "oracle_exact": oracle_exact},
+        "dropin_path": dropin,
         "device": {"cus": num_cus, "hbm_gb": round(hbm_total / 2**30, 1)},
         "setup_s": round(gen_s, 2),
     }

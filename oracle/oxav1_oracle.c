@@ -68,7 +68,7 @@ static inline void pb_set(oxav1_plane_i32 *b, int32_t y, int32_t x, int32_t v) {
  * every edge of the pass reads the pass-start samples (all edges "at once"),
  * while writes still land in the live plane.  Test infrastructure for the
  * sequential-vs-parallel equivalence of the in-place walk. */
-static const oxav1_plane_i32 *s_lf_read = NULL;
+static _Thread_local const oxav1_plane_i32 *s_lf_read = NULL;
 static inline int32_t lf_get(const oxav1_plane_i32 *b, int32_t y, int32_t x) {
     return pb_get(s_lf_read ? s_lf_read : b, y, x);
 }

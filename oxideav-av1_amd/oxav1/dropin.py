@@ -91,3 +91,56 @@ def gpu():
     if _gpu is None:
         _gpu = FilterCalls(abi.load(), GPU_NAMES, status_returning=True)
     return _gpu
+
+
+def frame_driver_chain(calls, frame, stage_s=None):
+    """The post-filter part of the reference's frame driver
+    (frame_driver.rs:790-1116) expressed with the five drop-in calls, the
+    way the decoder would drive them once bound to the C ABI: deblock in
+    place; keep a deblocked copy when loop restoration is on; CDEF from a
+    copy into the live planes; superres of the CDEF set and of the
+    deblocked set; loop restoration reading the deblocked set and a copy of
+    the CDEF set into the live planes; crop; film grain on a separate
+    output.  `calls` is a FilterCalls (the HIP library or the oracle).
+    Returns (pre-grain reference planes, output planes); stage_s, when a
+    list, receives the wall time of each of the five stages."""
+    import time
+    d = frame.desc
+    np_ = d.num_planes
+    times = []
+    t = time.perf_counter()
+    rec = [p.copy() for p in frame.planes]
+    if d.apply_lf:
+        rec = calls.loop_filter_frame(frame, rec)
+    times.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    uses_lr = bool(d.lr.uses_lr)
+    deb = rec
+    cur = calls.cdef_frame(frame, rec) if d.apply_cdef else rec
+    times.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    shapes = [frame.out_shape(p) for p in range(np_)]
+    if d.use_superres and d.upscaled_width > d.frame_width:
+        ctx = frame.sr_ctx()
+        st, upc = calls.upscale_frame(ctx, cur, shapes)
+        abi.check(st, "upscale_frame (CDEF set)")
+        upd = upc
+        if uses_lr:
+            st, upd = calls.upscale_frame(ctx, deb, shapes)
+            abi.check(st, "upscale_frame (deblocked set)")
+    else:
+        upc, upd = cur, deb
+    times.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    fin = calls.loop_restoration_frame(frame, upd, upc, lr_init=upc) if uses_lr else upc
+    times.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    ref = [np.ascontiguousarray(fin[p][:shapes[p][0], :shapes[p][1]]) for p in range(np_)]
+    out = ref
+    if d.fg.apply_grain:
+        out = calls.film_grain_synthesis(d.fg, d.bit_depth, np_, d.subsampling_x, d.subsampling_y,
+                                         d.matrix_coefficients, ref)
+    times.append(time.perf_counter() - t)
+    if stage_s is not None:
+        stage_s[:] = times
+    return ref, out

@@ -159,3 +159,23 @@ def test_film_grain_chroma_from_luma(gpu_calls, oracle_mod):
     args = (fg, 8, 3, 1, 1, 0)
     _eq(gpu_calls.film_grain_synthesis(*args, planes),
         oracle_mod.calls().film_grain_synthesis(*args, planes), "fg chroma_scaling_from_luma")
+
+
+@pytest.mark.parametrize("case", [
+    dict(cfg=0),
+    dict(kw=dict(bit_depth=10, noise_amp=32, superres_denom=12, upscaled_width=200, frame_height=96,
+                 enable_fg=1, fg_num_y_points=8, fg_ar_lag=3, fg_chroma_points=4, fg_overlap=1)),
+    dict(kw=dict(bit_depth=12, noise_amp=128, subsampling_y=0, superres_denom=10, upscaled_width=160,
+                 frame_height=70, lr_unit_shift=1)),
+    dict(kw=dict(mono=1, bit_depth=8, enable_fg=1, fg_num_y_points=5, fg_ar_lag=1)),
+], ids=["cfg0", "sr_fg_10bit", "sr_12bit_422", "mono_fg"])
+def test_frame_driver_through_dropin_calls(case, gpu_calls, oracle_mod):
+    """The frame driver's post-filter sequence (frame_driver.rs:790-1116)
+    driven through the five drop-in calls on the GPU equals the oracle's
+    chain: the boundary composes the way the decoder would use it."""
+    from oxav1 import Frame, dropin
+    frame = Frame.generate(case["cfg"], 0x0A1F0000) if "cfg" in case else _frame(21, **case["kw"])
+    ref_g, out_g = dropin.frame_driver_chain(gpu_calls, frame)
+    ref_o, out_o, _ = oracle_mod.chain(frame)
+    _eq(ref_g, ref_o, "pre-grain reference")
+    _eq(out_g, out_o, "film-grain output")

@@ -76,3 +76,23 @@ def test_deblock_parallel_differs_on_overlapping_edges(oracle_mod):
     seq, par = _lf_both(oracle_mod, f)
     assert not np.array_equal(seq[0], f.planes[0])
     assert not np.array_equal(seq[0], par[0])
+
+
+def test_frame_driver_mirror_equals_oracle_chain(oracle_mod):
+    """The Python frame-driver mirror over the oracle's five entry points
+    equals the oracle's own chain, so the GPU test of the mirror over the
+    drop-in calls compares like with like."""
+    from conftest import gen_config
+    from oxav1 import Frame, dropin
+    calls = oracle_mod.calls()
+    for kw, seed in ((dict(), 1),
+                     (dict(bit_depth=10, noise_amp=32, superres_denom=12, upscaled_width=160,
+                           frame_height=72, enable_fg=1, fg_num_y_points=8, fg_ar_lag=3,
+                           fg_chroma_points=4, fg_overlap=1), 2),
+                     (dict(mono=1, bit_depth=12, noise_amp=128), 3),
+                     (dict(enable_lr=0, superres_denom=9, upscaled_width=120, frame_height=64), 4)):
+        f = Frame.generate(gen_config(**kw), seed)
+        ref_o, out_o, _ = oracle_mod.chain(f)
+        ref_d, out_d = dropin.frame_driver_chain(calls, f)
+        for a, b in zip(ref_o + out_o, ref_d + out_d):
+            assert np.array_equal(a, b), kw
