@@ -404,15 +404,21 @@ typedef struct oxav1_tuning {
     uint16_t blocks_per_cu[5][2]; /* persistent workgroups per CU for lf, cdef, sr, lr, fg
                                      (0 = as many as fit); a batch uses the row of the
                                      sample size holding most of its pixels */
+    uint8_t waves_per_eu[5][2];   /* register budget of each stage's kernel: the compiled
+                                     variant that allows at least 6, 7 or 8 waves per SIMD
+                                     (0 = the compiler's choice) */
     uint8_t sr_rows_per_item[2];  /* superres rows per work item (0 = 8, at most 64) */
     uint8_t lr_tile_w[2];         /* loop restoration tile width: 32, or 64 where it divides
                                      the unit (0 = widest that divides) */
+    uint8_t sub_batches[2];       /* batched path split into this many sub-batches of balanced
+                                     pixels on streams of their own, so stages of different
+                                     frames overlap (0 or 1 = one batch, at most 4) */
 } oxav1_tuning;
 int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning *t);
 /* Current tuning, the CU count and how many workgroups of each stage fit
- * on one CU (any pointer may be NULL). */
+ * on one CU, per waves_per_eu variant (0, 6, 7, 8); any pointer may be NULL. */
 int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, uint32_t *num_cus,
-                               uint32_t fit_blocks_per_cu[5]);
+                               uint32_t fit_blocks_per_cu[5][4]);
 /* OXAV1_OK when everything enqueued on the engine's stream has finished,
  * OXAV1_ERR_BUSY while work is in flight; never blocks. */
 int oxav1_pf_engine_query(oxav1_pf_engine *e);

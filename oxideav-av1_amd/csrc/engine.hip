@@ -440,7 +440,15 @@ struct oxav1_pf_engine {
     int persistent_blocks[5][kWaveVariants] = {};  // workgroups per CU that fit, per stage and variant
     oxav1_tuning tuning = {};                       // zero = defaults
     int batch_kind = 1;                             // 0 = mostly 8-bit, 1 = mostly 10/12-bit
+    // Sub-batches: the batched path split over child engines on their own
+    // streams (the children hold job tables built here, no frame slots).
+    std::vector<oxav1_pf_engine *> subs;
+    std::vector<hipEvent_t> sub_join;
+    hipEvent_t sub_fork = nullptr;
+    uint32_t nsub = 0;
 };
+
+constexpr uint32_t kMaxSubBatches = 4;
 
 static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
 
@@ -717,6 +725,10 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
 extern "C" int oxav1_pf_engine_destroy(oxav1_pf_engine *e) {
     if (!e) return OXAV1_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto *c : e->subs) (void)oxav1_pf_engine_destroy(c);
+    for (auto ev : e->sub_join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->sub_fork) (void)hipEventDestroy(e->sub_fork);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (auto &s : e->slots) s.release();
@@ -733,6 +745,9 @@ extern "C" int oxav1_pf_engine_destroy(oxav1_pf_engine *e) {
     return OXAV1_OK;
 }
 
+static int load_batch(oxav1_pf_engine *e);
+static int split_batch(oxav1_pf_engine *e);
+
 extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_desc *desc, const oxav1_frame_dev *dev,
                                        uint32_t n) {
     if (!e || !desc || !dev || n == 0) return OXAV1_ERR_INVALID_ARG;
@@ -741,13 +756,22 @@ extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_des
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->slots.size() < n) e->slots.resize(n);
     e->jobs_h.resize(n);
-    e->any_fg = false;
+    e->nsub = 0;
     for (uint32_t f = 0; f < n; f++) {
         int st = build_job(e, desc[f], dev[f], e->slots[f], e->jobs_h[f]);
         if (st) return st;
-        e->any_fg |= e->jobs_h[f].apply_fg != 0;
     }
+    int st = load_batch(e);
+    if (st) return st;
+    return split_batch(e);
+}
+
+// Device job table and per-stage prefix tables of the batch in e->jobs_h.
+static int load_batch(oxav1_pf_engine *e) {
+    const uint32_t n = (uint32_t)e->jobs_h.size();
     e->n = n;
+    e->any_fg = false;
+    for (const FrameJob &J : e->jobs_h) e->any_fg |= J.apply_fg != 0;
     {
         uint64_t px[2] = {0, 0};
         for (uint32_t f = 0; f < n; f++)
@@ -775,6 +799,55 @@ extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_des
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->graph_exec = nullptr;
     e->graph = nullptr;
+    return OXAV1_OK;
+}
+
+// Split the prepared batch into sub-batches of balanced output pixels, one
+// per child engine, when the tuning asks for more than one: stages of
+// different frames then run concurrently on the children's streams, which
+// fills the issue slots one stage leaves idle (tails of the persistent
+// launches, latency-bound stages beside issue-bound ones).
+static int split_batch(oxav1_pf_engine *e) {
+    e->nsub = 0;
+    const uint32_t n = e->n;
+    const uint32_t want = std::min<uint32_t>(e->tuning.sub_batches[e->batch_kind], kMaxSubBatches);
+    if (want < 2 || n < 2) return OXAV1_OK;
+    const uint32_t K = std::min(want, n);
+    std::vector<uint32_t> order(n);
+    for (uint32_t f = 0; f < n; f++) order[f] = f;
+    auto px = [&](uint32_t f) { return (uint64_t)e->jobs_h[f].uw * e->jobs_h[f].fh * e->jobs_h[f].np; };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return px(a) > px(b); });
+    std::vector<std::vector<uint32_t>> grp(K);
+    std::vector<uint64_t> load(K, 0);
+    for (uint32_t f : order) {
+        uint32_t g = 0;
+        for (uint32_t k = 1; k < K; k++)
+            if (load[k] < load[g]) g = k;
+        grp[g].push_back(f);
+        load[g] += px(f);
+    }
+    if (!e->sub_fork && hipEventCreateWithFlags(&e->sub_fork, hipEventDisableTiming) != hipSuccess)
+        return OXAV1_ERR_HIP;
+    while (e->subs.size() < K) {
+        oxav1_pf_engine *c = nullptr;
+        int st = oxav1_pf_engine_create(&c, nullptr, e->max_frames);
+        if (st) return st;
+        e->subs.push_back(c);
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return OXAV1_ERR_HIP;
+        e->sub_join.push_back(ev);
+    }
+    for (uint32_t k = 0; k < K; k++) {
+        oxav1_pf_engine *c = e->subs[k];
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->tuning = e->tuning;
+        c->tuning.sub_batches[0] = c->tuning.sub_batches[1] = 0;
+        c->jobs_h.clear();
+        for (uint32_t f : grp[k]) c->jobs_h.push_back(e->jobs_h[f]);
+        int st = load_batch(c);
+        if (st) return st;
+    }
+    e->nsub = K;
     return OXAV1_OK;
 }
 
@@ -861,6 +934,18 @@ extern "C" int oxav1_pf_engine_run(oxav1_pf_engine *e, int mode) {
         int st = enqueue_plain(e, e->timing);
         if (st) return st;
         return e->timing ? collect_timing(e) : OXAV1_OK;
+    }
+    if ((mode == 1 || mode == 2) && !e->timing && e->nsub > 1) {
+        HIP_TRY(hipEventRecord(e->sub_fork, e->stream));
+        for (uint32_t k = 0; k < e->nsub; k++) {
+            oxav1_pf_engine *c = e->subs[k];
+            HIP_TRY(hipStreamWaitEvent(c->stream, e->sub_fork, 0));
+            int st = oxav1_pf_engine_run(c, mode);
+            if (st) return st;
+            HIP_TRY(hipEventRecord(e->sub_join[k], c->stream));
+            HIP_TRY(hipStreamWaitEvent(e->stream, e->sub_join[k], 0));
+        }
+        return OXAV1_OK;
     }
     if (mode == 1 || e->timing) {
         int st = enqueue_batch(e, e->timing);
@@ -972,7 +1057,7 @@ extern "C" int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning
     for (int k = 0; k < 2; k++) {
         for (int st = 0; st < 5; st++)
             if (t->waves_per_eu[st][k] && !wave_variant(t->waves_per_eu[st][k])) return OXAV1_ERR_INVALID_ARG;
-        if (t->sr_rows_per_item[k] > 64) return OXAV1_ERR_INVALID_ARG;
+        if (t->sr_rows_per_item[k] > 64 || t->sub_batches[k] > kMaxSubBatches) return OXAV1_ERR_INVALID_ARG;
         if (t->lr_tile_w[k] != 0 && t->lr_tile_w[k] != 32 && t->lr_tile_w[k] != 64) return OXAV1_ERR_INVALID_ARG;
     }
     e->tuning = *t;

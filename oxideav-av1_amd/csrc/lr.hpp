@@ -177,16 +177,36 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
         for (int i = tid; i < 256; i += nt) S.a2[i] = c_sgr_a2[i];
     __syncthreads();
     {
-        const uint32_t m = ((1u << 20) + (uint32_t)g.ww - 1) / (uint32_t)g.ww;
+        // Window rows go to the waves in turn, lanes to columns 0..63 and
+        // 64..ww-1, so the column clamps are computed once per lane.  Four
+        // rows' loads are issued before their stores (their latencies
+        // overlap); every load is in bounds: rows past the window repeat the
+        // last row (no store), rows without a buffer read the CDEF plane's
+        // first sample and store 0.
         const int32_t xmax = max(g.pex, 0);
-        const int n = g.wh * g.ww;
-#pragma unroll 4
-        for (int i = tid; i < n; i += nt) {
-            const int yy = (int)(((uint32_t)i * m) >> 20), xx = i - yy * g.ww;  // exact: i < 2^20 / 70
-            const int32_t cm = S.rowmax[yy];
-            const int32_t x = clip3(0, cm, clip3(0, xmax, g.wx0 + xx));
-            const G<T> *row = (const G<T> *)S.rowp[yy];
-            S.src[yy * kLrSw + xx] = cm < 0 ? (L)0 : (L)row[x];
+        const int32_t xa = clip3(0, xmax, g.wx0 + lane);
+        const int32_t xb = clip3(0, xmax, g.wx0 + lane + 64);
+        const bool st_a = lane < g.ww, st_b = lane + 64 < g.ww;
+        const int nrw = (g.wh - wv + nwv - 1) / nwv;  // this wave's rows (uniform in the wave)
+        for (int i0 = 0; i0 < nrw; i0 += 4) {
+            L va[4], vb[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int rr = min(wv + (i0 + j) * nwv, g.wh - 1);
+                const int32_t cm = S.rowmax[rr];
+                const G<T> *row = cm < 0 ? cdf : (const G<T> *)S.rowp[rr];
+                const int32_t c = max(cm, 0);
+                va[j] = cm < 0 ? (L)0 : (L)row[min(xa, c)];
+                vb[j] = cm < 0 ? (L)0 : (L)row[min(xb, c)];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = wv + (i0 + j) * nwv;
+                if (i0 + j < nrw) {
+                    if (st_a) S.src[r * kLrSw + lane] = va[j];
+                    if (st_b) S.src[r * kLrSw + lane + 64] = vb[j];
+                }
+            }
         }
     }
     __syncthreads();

@@ -8,6 +8,8 @@ batched persistent path) over a sweep of
   - persistent workgroups per CU (1 .. as many as fit),
   - superres rows per work item (4, 8, 16, 32),
   - loop-restoration tile width (32, 64),
+  - sub-batches on streams of their own (1-4; timed on whole graph-replayed
+    steps, since concurrent stages cannot be timed one by one),
 one knob at a time, keeps the fastest (a candidate must beat the current
 choice by more than 1.5% to replace it), and checks that the outputs of the
 chosen shape are byte-identical to those of the default shape (no knob
@@ -39,6 +41,7 @@ def to_struct(table):
     for k, kind in enumerate(KINDS):
         t.sr_rows_per_item[k] = int(table.get("sr_rows_per_item", {}).get(kind, 0))
         t.lr_tile_w[k] = int(table.get("lr_tile_w", {}).get(kind, 0))
+        t.sub_batches[k] = int(table.get("sub_batches", {}).get(kind, 0))
     return t
 
 
@@ -60,10 +63,22 @@ def _stage_ms(eng, stage, steps):
     return ms
 
 
-def _outputs(eng, dfs):
+def _step_ms(eng, steps):
+    """Whole batched step (hipGraph replay), ms."""
+    for _ in range(2):
+        eng.run(2)
+    eng.sync()
+    t = time.perf_counter()
+    for _ in range(steps * 2):
+        eng.run(2)
+    eng.sync()
+    return (time.perf_counter() - t) / (steps * 2) * 1e3
+
+
+def _outputs(eng, dfs, mode=1):
     for df in dfs:
         df.clear_outputs()
-    eng.run(1)
+    eng.run(mode)
     eng.sync()
     return [df.result() for df in dfs]
 
@@ -149,11 +164,24 @@ def tune_kind(dfs, kind, steps=6, log=print):
             t.lr_tile_w[k] = lr_best
             timings["lr_tile_w"] = {str(c): round(v, 4) for c, v in res.items()}
             log(f"  {kind} lr tile width {res} -> {lr_best}")
+        # sub-batches overlap stages of different frames: timed on whole
+        # batched steps (graph replay), since per-stage events cannot
+        # separate concurrent stages
+        res = {}
+        for nsub in (1, 2, 3, 4):
+            t.sub_batches[k] = nsub
+            eng.set_tuning(t)
+            eng.prepare(dfs)
+            res[nsub] = _step_ms(eng, steps)
+        sub_best = _pick(res, 1)
+        t.sub_batches[k] = sub_best
+        timings["sub_batches_step_ms"] = {str(c): round(v, 4) for c, v in res.items()}
+        log(f"  {kind} sub-batches (whole step ms) {res} -> {sub_best}")
         eng.set_tuning(t)
         eng.prepare(dfs)
-        if not _same(_outputs(eng, dfs), ref):
+        if not _same(_outputs(eng, dfs, 2), ref):
             raise RuntimeError(f"{kind}: tuned launch shape changed the output")
-        return best_w, best_bpc, sr_best, lr_best, timings
+        return best_w, best_bpc, sr_best, lr_best, sub_best, timings
     finally:
         eng.close()
 
@@ -189,9 +217,10 @@ def main():
     from .engine import device_info
     t0 = time.time()
     table = {"device": {"cus": device_info()[0]}, "waves_per_eu": {}, "blocks_per_cu": {},
-             "sr_rows_per_item": {}, "lr_tile_w": {}, "timings_ms": {}}
+             "sr_rows_per_item": {}, "lr_tile_w": {}, "sub_batches": {}, "timings_ms": {}}
     for kind, dfs in workloads(args.frames).items():
-        wv, bpc, sr, lr, tm = tune_kind(dfs, kind, args.steps)
+        wv, bpc, sr, lr, sub, tm = tune_kind(dfs, kind, args.steps)
+        table["sub_batches"][kind] = sub
         table["waves_per_eu"][kind] = wv
         table["blocks_per_cu"][kind] = bpc
         table["sr_rows_per_item"][kind] = sr

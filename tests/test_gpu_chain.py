@@ -97,17 +97,18 @@ def test_chain_4k_10bit_superres(oracle_mod):
 
 
 @pytest.mark.parametrize("knobs", [
-    dict(blocks=1, sr_rows=4, lr_w=32, waves=0),
-    dict(blocks=2, sr_rows=16, lr_w=64, waves=6),
-    dict(blocks=0, sr_rows=32, lr_w=32, waves=7),
-    dict(blocks=3, sr_rows=5, lr_w=0, waves=8),
-    dict(blocks=0, sr_rows=0, lr_w=0, waves=8),
+    dict(blocks=1, sr_rows=4, lr_w=32, waves=0, sub=0),
+    dict(blocks=2, sr_rows=16, lr_w=64, waves=6, sub=2),
+    dict(blocks=0, sr_rows=32, lr_w=32, waves=7, sub=3),
+    dict(blocks=3, sr_rows=5, lr_w=0, waves=8, sub=4),
+    dict(blocks=0, sr_rows=0, lr_w=0, waves=8, sub=2),
 ])
 def test_chain_tuned_launch_shapes(knobs, oracle_mod):
     """The autotuner's knobs (register-budget variant, persistent workgroups
-    per CU, superres rows per item, loop-restoration tile width) leave every
-    output byte as the oracle's, for 8-bit and 10/12-bit frames in one
-    batch, on the batched and the plain path."""
+    per CU, superres rows per item, loop-restoration tile width, sub-batches
+    on their own streams) leave every output byte as the oracle's, for 8-bit
+    and 10/12-bit frames in one batch, on the batched path, its graph
+    replays and the plain path."""
     from oxav1 import abi
     from oxav1.engine import Engine
     frames = _frames([(gen_config(**kw), s) for kw, s in SMALL])
@@ -119,11 +120,18 @@ def test_chain_tuned_launch_shapes(knobs, oracle_mod):
     for k in range(2):
         t.sr_rows_per_item[k] = knobs["sr_rows"]
         t.lr_tile_w[k] = knobs["lr_w"]
+        t.sub_batches[k] = knobs["sub"]
     eng = Engine(max_frames=len(frames), tuning=t)
     eng.prepare(frames)
     eng.run(1)
     eng.sync()
     _check(frames, oracle_mod, f"tuning {knobs}")
+    for _ in range(3):  # graph replays (one graph per sub-batch), outputs cleared between
+        for df in frames:
+            df.clear_outputs()
+        eng.run(2)
+        eng.sync()
+        _check(frames, oracle_mod, f"tuning {knobs}, graph replay")
     for df in frames:
         df.clear_outputs()
         eng.prepare([df])
