@@ -413,6 +413,9 @@ typedef struct oxav1_tuning {
     uint8_t sub_batches[2];       /* batched path split into this many sub-batches of balanced
                                      pixels on streams of their own, so stages of different
                                      frames overlap (0 or 1 = one batch, at most 4) */
+    uint8_t items_per_claim[5][2]; /* work items a persistent workgroup takes per atomic on
+                                     the stage's counter: 0 or 1 = one, up to 16, or 255 =
+                                     no atomics (items blockIdx.x + k * gridDim.x) */
 } oxav1_tuning;
 int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning *t);
 /* Current tuning, the CU count and how many workgroups of each stage fit

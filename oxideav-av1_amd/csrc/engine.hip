@@ -132,19 +132,31 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
 }
 
 // --- kernels: KIND 0 = engine (u8/u16 by FrameJob::kind), 1 = int32 ---
-// counter == nullptr: one workgroup per item (blockIdx.x); else persistent.
+// counter == nullptr: one workgroup per item (blockIdx.x).  Else persistent:
+// claim >= 1 items are taken per atomic on the stage counter (all
+// workgroups contend for that one address, so stages with many cheap items
+// claim several at once), claim == 0 walks items blockIdx.x + k * gridDim.x
+// without atomics.
 #define OXAV1_ITEM_LOOP(BODY)                                                     \
     __shared__ uint32_t s_item;                                                   \
+    uint32_t base_ = 0, left_ = 0;                                                \
     for (uint32_t iter = 0;; iter++) {                                            \
         uint32_t item;                                                            \
-        if (counter) {                                                            \
-            if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);                \
-            __syncthreads();                                                      \
-            item = s_item;                                                        \
-            __syncthreads();                                                      \
-        } else {                                                                  \
+        if (!counter) {                                                           \
             if (iter > 0) break;                                                  \
             item = blockIdx.x;                                                    \
+        } else if (claim == 0) {                                                  \
+            item = blockIdx.x + iter * gridDim.x;                                 \
+        } else {                                                                  \
+            if (left_ == 0) {                                                     \
+                if (threadIdx.x == 0) s_item = atomicAdd(counter, claim);         \
+                __syncthreads();                                                  \
+                base_ = s_item;                                                   \
+                left_ = claim;                                                    \
+                __syncthreads();                                                  \
+            }                                                                     \
+            item = base_++;                                                       \
+            left_--;                                                              \
         }                                                                         \
         if (item >= total) break;                                                 \
         const ItemRef ref = find_item(prefix, njobs, item);                       \
@@ -155,7 +167,7 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
 
 template <int KIND>
 __device__ __forceinline__ void lf_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                 uint32_t total, uint32_t *counter) {
+                                                 uint32_t total, uint32_t *counter, uint32_t claim) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L region[kLfRegion * kLfStride];
     __shared__ LfShared sh;
@@ -173,7 +185,7 @@ __device__ __forceinline__ void lf_body(const FrameJob *jobs, const uint32_t *pr
 
 template <int KIND>
 __device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                 uint32_t total, uint32_t *counter) {
+                                                 uint32_t total, uint32_t *counter, uint32_t claim) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ L tile[kCdefLw * kCdefLw];
     __shared__ CdefBlk blk[64];
@@ -193,7 +205,7 @@ __device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *
 
 template <int KIND>
 __device__ __forceinline__ void sr_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                 uint32_t total, uint32_t *counter) {
+                                                 uint32_t total, uint32_t *counter, uint32_t claim) {
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
             sr_item<int32_t>(J, ref.local);
@@ -208,7 +220,7 @@ __device__ __forceinline__ void sr_body(const FrameJob *jobs, const uint32_t *pr
 
 template <int KIND>
 __device__ __forceinline__ void lr_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                 uint32_t total, uint32_t *counter) {
+                                                 uint32_t total, uint32_t *counter, uint32_t claim) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
     __shared__ LrSmem<L> S;
     OXAV1_ITEM_LOOP({
@@ -225,7 +237,7 @@ __device__ __forceinline__ void lr_body(const FrameJob *jobs, const uint32_t *pr
 
 template <int KIND>
 __device__ __forceinline__ void fg_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
-                                                 uint32_t total, uint32_t *counter) {
+                                                 uint32_t total, uint32_t *counter, uint32_t claim) {
     __shared__ uint8_t lut[768];
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
@@ -246,13 +258,13 @@ __device__ __forceinline__ void fg_body(const FrameJob *jobs, const uint32_t *pr
 #define OXAV1_STAGE_KERNELS(NAME)                                                                         \
     template <int KIND>                                                                                   \
     __global__ __launch_bounds__(kThreads) void k_##NAME(const FrameJob *jobs, const uint32_t *prefix,   \
-                                                         uint32_t njobs, uint32_t total, uint32_t *counter) { \
-        NAME##_body<KIND>(jobs, prefix, njobs, total, counter);                                          \
+                                                         uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t claim) { \
+        NAME##_body<KIND>(jobs, prefix, njobs, total, counter, claim);                                          \
     }                                                                                                     \
     template <int W>                                                                                      \
     __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W))) void k_##NAME##_w(    \
-        const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t total, uint32_t *counter) { \
-        NAME##_body<0>(jobs, prefix, njobs, total, counter);                                             \
+        const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t claim) { \
+        NAME##_body<0>(jobs, prefix, njobs, total, counter, claim);                                             \
     }
 OXAV1_STAGE_KERNELS(lf)
 OXAV1_STAGE_KERNELS(cdef)
@@ -380,7 +392,7 @@ static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw, uint32_t pref =
 
 static uint32_t lr_stripe_count(uint32_t pey, uint32_t sy) { return ((pey << sy) + 8) / 64 + 1; }
 
-using StageKernel = void (*)(const FrameJob *, const uint32_t *, uint32_t, uint32_t, uint32_t *);
+using StageKernel = void (*)(const FrameJob *, const uint32_t *, uint32_t, uint32_t, uint32_t *, uint32_t);
 constexpr int kWaveVariants = 4;
 constexpr int kWaveOf[kWaveVariants] = {0, 6, 7, 8};  // 0 = the compiler's choice
 #define OXAV1_VARIANTS(K) {K<0>, K##_w<6>, K##_w<7>, K##_w<8>}
@@ -449,6 +461,7 @@ struct oxav1_pf_engine {
 };
 
 constexpr uint32_t kMaxSubBatches = 4;
+constexpr uint8_t kClaimStatic = 255;  // items_per_claim value: no atomics, grid-stride items
 
 static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
 
@@ -871,7 +884,7 @@ static int enqueue_plain(oxav1_pf_engine *e, bool timed) {
             if (items) {
                 const int w = wave_variant(e->tuning.waves_per_eu[s][J.kind == kU8 ? 0 : 1]);
                 hipLaunchKernelGGL(kEngineKernels[s][w], dim3(items), dim3(kThreads), 0, e->stream, jd + f,
-                                   (const uint32_t *)nullptr, 1u, items, (uint32_t *)nullptr);
+                                   (const uint32_t *)nullptr, 1u, items, (uint32_t *)nullptr, 1u);
                 HIP_TRY(hipGetLastError());
             }
             if (timed && f == 0) HIP_TRY(hipEventRecord(e->ev[s + 1], e->stream));
@@ -902,12 +915,14 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
         if (s == kStageFg && e->any_fg) HIP_TRY(hipStreamWaitEvent(e->stream, e->join, 0));
         if (total) {
             const int w = wave_variant(e->tuning.waves_per_eu[s][e->batch_kind]);
+            const uint8_t cl = e->tuning.items_per_claim[s][e->batch_kind];
+            const uint32_t claim = cl == 0 ? 1u : (cl == kClaimStatic ? 0u : cl);
             const uint32_t per_cu = e->tuning.blocks_per_cu[s][e->batch_kind];
             const uint32_t fit = (uint32_t)e->persistent_blocks[s][w];
             const uint32_t grid =
                 std::min<uint32_t>(total, (uint32_t)e->num_cus * (per_cu ? std::min(per_cu, fit) : fit));
             hipLaunchKernelGGL(kEngineKernels[s][w], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
-                               e->n, total, ctr + s);
+                               e->n, total, ctr + s, claim);
             HIP_TRY(hipGetLastError());
         }
         if (timed) HIP_TRY(hipEventRecord(e->ev[s + 1], e->stream));
@@ -1058,6 +1073,9 @@ extern "C" int oxav1_pf_engine_set_tuning(oxav1_pf_engine *e, const oxav1_tuning
         for (int st = 0; st < 5; st++)
             if (t->waves_per_eu[st][k] && !wave_variant(t->waves_per_eu[st][k])) return OXAV1_ERR_INVALID_ARG;
         if (t->sr_rows_per_item[k] > 64 || t->sub_batches[k] > kMaxSubBatches) return OXAV1_ERR_INVALID_ARG;
+        for (int st = 0; st < 5; st++)
+            if (t->items_per_claim[st][k] > 16 && t->items_per_claim[st][k] != kClaimStatic)
+                return OXAV1_ERR_INVALID_ARG;
         if (t->lr_tile_w[k] != 0 && t->lr_tile_w[k] != 32 && t->lr_tile_w[k] != 64) return OXAV1_ERR_INVALID_ARG;
     }
     e->tuning = *t;

@@ -158,7 +158,8 @@ class GenFrame(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", (C.c_uint16 * 2) * 5), ("waves_per_eu", (u8 * 2) * 5),
-                ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2), ("sub_batches", u8 * 2)]
+                ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2), ("sub_batches", u8 * 2),
+                ("items_per_claim", (u8 * 2) * 5)]
 
 
 class ServerConfig(C.Structure):

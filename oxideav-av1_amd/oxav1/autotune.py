@@ -6,6 +6,8 @@ batched persistent path) over a sweep of
   - the kernel's register budget: the compiled variant that allows at least
     6, 7 or 8 waves per SIMD, or the compiler's own choice,
   - persistent workgroups per CU (1 .. as many as fit),
+  - work items claimed per atomic on the stage counter (1, 2, 4, 8, or a
+    static grid-stride walk without atomics),
   - superres rows per work item (4, 8, 16, 32),
   - loop-restoration tile width (32, 64),
   - sub-batches on streams of their own (1-4; timed on whole graph-replayed
@@ -38,6 +40,7 @@ def to_struct(table):
         for k, kind in enumerate(KINDS):
             t.blocks_per_cu[s][k] = int(table.get("blocks_per_cu", {}).get(kind, {}).get(name, 0))
             t.waves_per_eu[s][k] = int(table.get("waves_per_eu", {}).get(kind, {}).get(name, 0))
+            t.items_per_claim[s][k] = int(table.get("items_per_claim", {}).get(kind, {}).get(name, 0))
     for k, kind in enumerate(KINDS):
         t.sr_rows_per_item[k] = int(table.get("sr_rows_per_item", {}).get(kind, 0))
         t.lr_tile_w[k] = int(table.get("lr_tile_w", {}).get(kind, 0))
@@ -89,6 +92,7 @@ def _same(a, b):
 
 MARGIN = 0.985
 WAVES = (0, 6, 7, 8)
+CLAIM_STATIC = 255
 
 
 def _pick(res, current):
@@ -114,7 +118,7 @@ def tune_kind(dfs, kind, steps=6, log=print):
         _, cus, fit = eng.tuning()
         active = [s for s in STAGES if eng.stage_items(s)]
         timings = {}
-        best_w, best_bpc = {}, {}
+        best_w, best_bpc, best_claim = {}, {}, {}
 
         def measure(stage):
             eng.set_tuning(t)
@@ -144,6 +148,15 @@ def tune_kind(dfs, kind, steps=6, log=print):
             best_bpc[s] = c
             timings[f"{s}_blocks_per_cu"] = {str(c): round(v, 4) for c, v in res.items()}
             log(f"  {kind} {s}: blocks/CU (0 = all {nfit} that fit) {res} -> {c}")
+            res = {1: res[c]}
+            for cl in (2, 4, 8, CLAIM_STATIC):
+                t.items_per_claim[si][k] = cl
+                res[cl] = measure(s)
+            cl = _pick(res, 1)
+            t.items_per_claim[si][k] = cl
+            best_claim[s] = cl
+            timings[f"{s}_items_per_claim"] = {str(c): round(v, 4) for c, v in res.items()}
+            log(f"  {kind} {s}: items per claim (255 = static) {res} -> {cl}")
         sr_best = 0
         if "sr" in active:
             res = {}
@@ -181,7 +194,7 @@ def tune_kind(dfs, kind, steps=6, log=print):
         eng.prepare(dfs)
         if not _same(_outputs(eng, dfs, 2), ref):
             raise RuntimeError(f"{kind}: tuned launch shape changed the output")
-        return best_w, best_bpc, sr_best, lr_best, sub_best, timings
+        return best_w, best_bpc, best_claim, sr_best, lr_best, sub_best, timings
     finally:
         eng.close()
 
@@ -217,9 +230,11 @@ def main():
     from .engine import device_info
     t0 = time.time()
     table = {"device": {"cus": device_info()[0]}, "waves_per_eu": {}, "blocks_per_cu": {},
-             "sr_rows_per_item": {}, "lr_tile_w": {}, "sub_batches": {}, "timings_ms": {}}
+             "items_per_claim": {}, "sr_rows_per_item": {}, "lr_tile_w": {}, "sub_batches": {},
+             "timings_ms": {}}
     for kind, dfs in workloads(args.frames).items():
-        wv, bpc, sr, lr, sub, tm = tune_kind(dfs, kind, args.steps)
+        wv, bpc, claim, sr, lr, sub, tm = tune_kind(dfs, kind, args.steps)
+        table["items_per_claim"][kind] = claim
         table["sub_batches"][kind] = sub
         table["waves_per_eu"][kind] = wv
         table["blocks_per_cu"][kind] = bpc

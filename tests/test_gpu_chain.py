@@ -97,16 +97,16 @@ def test_chain_4k_10bit_superres(oracle_mod):
 
 
 @pytest.mark.parametrize("knobs", [
-    dict(blocks=1, sr_rows=4, lr_w=32, waves=0, sub=0),
-    dict(blocks=2, sr_rows=16, lr_w=64, waves=6, sub=2),
-    dict(blocks=0, sr_rows=32, lr_w=32, waves=7, sub=3),
-    dict(blocks=3, sr_rows=5, lr_w=0, waves=8, sub=4),
-    dict(blocks=0, sr_rows=0, lr_w=0, waves=8, sub=2),
+    dict(blocks=1, sr_rows=4, lr_w=32, waves=0, sub=0, claim=0),
+    dict(blocks=2, sr_rows=16, lr_w=64, waves=6, sub=2, claim=3),
+    dict(blocks=0, sr_rows=32, lr_w=32, waves=7, sub=3, claim=255),
+    dict(blocks=3, sr_rows=5, lr_w=0, waves=8, sub=4, claim=16),
+    dict(blocks=0, sr_rows=0, lr_w=0, waves=8, sub=2, claim=2),
 ])
 def test_chain_tuned_launch_shapes(knobs, oracle_mod):
     """The autotuner's knobs (register-budget variant, persistent workgroups
-    per CU, superres rows per item, loop-restoration tile width, sub-batches
-    on their own streams) leave every output byte as the oracle's, for 8-bit
+    per CU, items per claim, superres rows per item, loop-restoration tile
+    width, sub-batches on their own streams) leave every output byte as the oracle's, for 8-bit
     and 10/12-bit frames in one batch, on the batched path, its graph
     replays and the plain path."""
     from oxav1 import abi
@@ -117,6 +117,7 @@ def test_chain_tuned_launch_shapes(knobs, oracle_mod):
         for k in range(2):
             t.blocks_per_cu[s][k] = knobs["blocks"]
             t.waves_per_eu[s][k] = knobs["waves"]
+            t.items_per_claim[s][k] = knobs["claim"]
     for k in range(2):
         t.sr_rows_per_item[k] = knobs["sr_rows"]
         t.lr_tile_w[k] = knobs["lr_w"]
