@@ -140,6 +140,7 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
 #define OXAV1_ITEM_LOOP(BODY)                                                     \
     __shared__ uint32_t s_item;                                                   \
     uint32_t base_ = 0, left_ = 0;                                                \
+    uint32_t cf_ = 0, clo_ = 0, chi_ = 0; /* items [clo_, chi_) are frame cf_ */  \
     for (uint32_t iter = 0;; iter++) {                                            \
         uint32_t item;                                                            \
         if (!counter) {                                                           \
@@ -159,7 +160,17 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
             left_--;                                                              \
         }                                                                         \
         if (item >= total) break;                                                 \
-        const ItemRef ref = find_item(prefix, njobs, item);                       \
+        ItemRef ref;                                                              \
+        if (!prefix) {                                                            \
+            ref = {0u, item};                                                     \
+        } else if (item - clo_ < chi_ - clo_) { /* same frame as the last item */ \
+            ref = {cf_, item - clo_};                                             \
+        } else {                                                                  \
+            ref = find_item(prefix, njobs, item);                                 \
+            cf_ = ref.frame;                                                      \
+            clo_ = item - ref.local;                                              \
+            chi_ = prefix[ref.frame + 1];                                         \
+        }                                                                         \
         const FrameJob &J = jobs[ref.frame];                                      \
         BODY;                                                                     \
         __syncthreads();                                                          \
