@@ -32,77 +32,80 @@ struct CdefBlk {
     int16_t pri_y, sec_y, pri_uv, sec_uv;
 };
 
-// cdef_direction (cdef.rs:366-449), costs of one half of the directions for
-// one 8x8 block read from the LDS tile: half 0 -> directions 0, 2, 4, 6;
-// half 1 -> directions 1, 3, 5, 7 (stored at cost[d]).  The block rows and
-// columns are clamped to the staged (buffer-bounded) extent lh x lw, as the
-// reference clamps its reads to the plane (cdef.rs:386-391).
-template <typename L>
-__device__ void cdef_dir_costs(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift, int half,
-                               S3<int32_t> *cost) {
+// cdef_direction (cdef.rs:366-449), costs of two of the eight directions
+// for one 8x8 block read from the LDS tile: quarter 0 -> directions 0 and 2,
+// 1 -> 4 and 6, 2 -> 1 and 3, 3 -> 5 and 7 (stored at cost[d]).  Four threads
+// share a block so each keeps at most 26 partial sums in registers.  The
+// block rows and columns are clamped to the staged (buffer-bounded) extent
+// lh x lw, as the reference clamps its reads to the plane (cdef.rs:386-391).
+template <typename L, int Q>
+__device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
+                                                 S3<int32_t> *cost) {
     auto px = [&](int i, int j) -> int32_t {
         const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
         return ((int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift) - 128;
     };
-    if (half == 0) {
-        int32_t p0[15], p4[15], p2[8], p6[8];
+    if constexpr (Q < 2) {
+        // Q 0: d = 0 (i + j) and d = 2 (rows); Q 1: d = 4 (7 + i - j) and
+        // d = 6 (columns).  Diagonals: 15 partial sums, outer pairs weighted
+        // by Div_Table.
+        int32_t pd[15], pa[8];
 #pragma unroll
-        for (int k = 0; k < 15; k++) p0[k] = p4[k] = 0;
+        for (int k = 0; k < 15; k++) pd[k] = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) p2[k] = p6[k] = 0;
+        for (int k = 0; k < 8; k++) pa[k] = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int32_t v = px(i, j);
-                p0[i + j] += v;
-                p4[7 + i - j] += v;
-                p2[i] += v;
-                p6[j] += v;
+                pd[Q == 0 ? i + j : 7 + i - j] += v;
+                pa[Q == 0 ? i : j] += v;
             }
-        int32_t c2 = 0, c6 = 0, c0 = 0, c4 = 0;
+        int32_t cd = pd[7] * pd[7] * 105, ca = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            c2 += p2[k] * p2[k];
-            c6 += p6[k] * p6[k];
-        }
+        for (int k = 0; k < 7; k++) cd += (pd[k] * pd[k] + pd[14 - k] * pd[14 - k]) * c_div_table[k + 1];
 #pragma unroll
-        for (int i = 0; i < 7; i++) {
-            c0 += (p0[i] * p0[i] + p0[14 - i] * p0[14 - i]) * c_div_table[i + 1];
-            c4 += (p4[i] * p4[i] + p4[14 - i] * p4[14 - i]) * c_div_table[i + 1];
-        }
-        c0 += p0[7] * p0[7] * 105;
-        c4 += p4[7] * p4[7] * 105;
-        cost[0] = c0;
-        cost[2] = c2 * 105;
-        cost[4] = c4;
-        cost[6] = c6 * 105;
+        for (int k = 0; k < 8; k++) ca += pa[k] * pa[k];
+        cost[Q == 0 ? 0 : 4] = cd;
+        cost[Q == 0 ? 2 : 6] = ca * 105;
     } else {
-        int32_t p1[11], p3[11], p5[11], p7[11];
+        // Q 2: d = 1 (i + j/2) and d = 3 (3 + i - j/2); Q 3: d = 5
+        // (3 - i/2 + j) and d = 7 (i/2 + j).  11 partial sums, the middle
+        // five weighted 105.
+        int32_t pa[11], pb[11];
 #pragma unroll
-        for (int k = 0; k < 11; k++) p1[k] = p3[k] = p5[k] = p7[k] = 0;
+        for (int k = 0; k < 11; k++) pa[k] = pb[k] = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int32_t v = px(i, j);
-                p1[i + j / 2] += v;
-                p3[3 + i - j / 2] += v;
-                p5[3 - i / 2 + j] += v;
-                p7[i / 2 + j] += v;
+                pa[Q == 2 ? i + j / 2 : 3 - i / 2 + j] += v;
+                pb[Q == 2 ? 3 + i - j / 2 : i / 2 + j] += v;
             }
-        int32_t *ps[4] = {p1, p3, p5, p7};
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-            const int32_t *p = ps[d];
+        auto half_cost = [&](const int32_t *p) {
             int32_t c = 0;
 #pragma unroll
-            for (int j = 0; j < 5; j++) c += p[3 + j] * p[3 + j];
+            for (int k = 0; k < 5; k++) c += p[3 + k] * p[3 + k];
             c *= 105;
 #pragma unroll
-            for (int j = 0; j < 3; j++) c += (p[j] * p[j] + p[10 - j] * p[10 - j]) * c_div_table[2 * j + 2];
-            cost[2 * d + 1] = c;
-        }
+            for (int k = 0; k < 3; k++) c += (p[k] * p[k] + p[10 - k] * p[10 - k]) * c_div_table[2 * k + 2];
+            return c;
+        };
+        cost[Q == 2 ? 1 : 5] = half_cost(pa);
+        cost[Q == 2 ? 3 : 7] = half_cost(pb);
+    }
+}
+
+template <typename L>
+__device__ __forceinline__ void cdef_dir_costs(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
+                                               int quarter, S3<int32_t> *cost) {
+    switch (quarter) {  // uniform per wave
+    case 0: cdef_dir_quarter<L, 0>(t, ls, lh, lw, y8, x8, shift, cost); break;
+    case 1: cdef_dir_quarter<L, 1>(t, ls, lh, lw, y8, x8, shift, cost); break;
+    case 2: cdef_dir_quarter<L, 2>(t, ls, lh, lw, y8, x8, shift, cost); break;
+    default: cdef_dir_quarter<L, 3>(t, ls, lh, lw, y8, x8, shift, cost); break;
     }
 }
 
@@ -271,8 +274,8 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
     cdef_load<L, T>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
     __syncthreads();
     // per 8x8 block: skip test (cdef.rs:266-272, out-of-grid cells count as
-    // skip) and the direction search, split over two waves (half 0: directions
-    // 0/2/4/6, half 1: 1/3/5/7) that meet through `costs`
+    // skip) and the direction search, split over the four waves (two
+    // directions each) that meet through `costs`
     const int nbx = (tw0 + 7) >> 3, nby = (th0 + 7) >> 3;
     auto block_active = [&](int b) -> bool {
         const int by = b >> 3, bx = b & 7;
@@ -287,11 +290,11 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
                           (r + 1 >= J.mi_rows || c + 1 >= J.mi_cols || skip_at(r + 1, c + 1));
         return !skip;
     };
-    if (tid < 128) {
-        const int b = tid & 63, half = tid >> 6;
-        if (block_active(b))
-            cdef_dir_costs<L>((const S3<L> *)t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, half,
-                             (S3<int32_t> *)(costs + b * 8));
+    {
+        const int b = tid & 63, quarter = tid >> 6;  // a wave per pair of directions
+        if (quarter < 4 && block_active(b))
+            cdef_dir_costs<L>((const S3<L> *)t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, quarter,
+                              (S3<int32_t> *)(costs + b * 8));
     }
     __syncthreads();
     if (tid < 64) {
