@@ -366,6 +366,32 @@ typedef struct oxav1_frame_dev {
     const oxav1_lr_unit *lr_units[3];
 } oxav1_frame_dev;
 
+/* The block walker's per-mi grids (device pointers, one byte per mi cell,
+ * row-major MiRows x MiCols), as cdf.rs:20320-20678 reads them for the
+ * loop-filter closures: is_inter, skip, RefFrame[0], YMode, segment_id,
+ * MiSize, TxSizes and InterTxSizes. */
+typedef struct oxav1_walker_grids {
+    const uint8_t *is_inter;
+    const uint8_t *skip;
+    const uint8_t *ref_frame;
+    const uint8_t *mode;
+    const uint8_t *segment_id;
+    const uint8_t *mi_size;
+    const uint8_t *tx_size;
+    const uint8_t *inter_tx_size;
+} oxav1_walker_grids;
+
+/* Build the engine's side information on the device from the walker's
+ * grids, asynchronously on hip_stream (NULL = the default stream): lf_mi
+ * (MiRows*MiCols packed words, OXAV1_MI_* layout) and lf_tx_size[p]
+ * (loopfilter_tx_size_at, cdf.rs:20559-20593: luma from InterTxSizes for
+ * inter cells and TxSizes for intra cells; chroma by get_tx_size of the
+ * co-located luma mi, cdf.rs:12976-13012, at ((MiRows+sy)>>sy) x
+ * ((MiCols+sx)>>sx)).  Replaces the host-side closure sampling. */
+int oxav1_pf_build_grids(const oxav1_walker_grids *g, uint32_t mi_rows, uint32_t mi_cols, uint8_t num_planes,
+                         uint8_t subsampling_x, uint8_t subsampling_y, uint32_t *lf_mi,
+                         uint8_t *const lf_tx_size[3], void *hip_stream);
+
 typedef struct oxav1_pf_engine oxav1_pf_engine;
 
 /* Create an engine bound to the current device and a HIP stream (NULL =
@@ -512,6 +538,14 @@ int oxav1_pf_server_slot(oxav1_pf_server *s, uint32_t session, uint32_t slot, ui
  * queued or in flight. */
 int oxav1_pf_server_submit(oxav1_pf_server *s, uint32_t session, const oxav1_frame_desc *desc,
                            const oxav1_frame_dev *dev, uint32_t slot, double arrival_us, uint64_t *ticket);
+/* Queue the show_existing_frame output of a slot (frame_driver.rs:1489-1537):
+ * film grain with `fg` (the slot's saved grain parameters) applied to the
+ * slot's stored pre-grain frame, written to the slot's grain buffer; the
+ * stored frame is not modified.  The session must have been opened with a
+ * grain shape (fg.apply_grain set) so that its slots carry grain buffers.
+ * Without grain (fg->apply_grain == 0) the stored frame is copied. */
+int oxav1_pf_server_submit_existing(oxav1_pf_server *s, uint32_t session, uint32_t slot,
+                                    const oxav1_film_grain_params *fg, double arrival_us, uint64_t *ticket);
 /* Retire finished lanes, launch ready frames on free lanes, and return up
  * to cap finished frames (the rest are kept for the next poll). */
 int oxav1_pf_server_poll(oxav1_pf_server *s, oxav1_server_done *done, uint32_t cap, uint32_t *n_done);

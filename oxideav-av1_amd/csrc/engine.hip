@@ -124,7 +124,8 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
     const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
     const uint32_t t = local - J.item_base[kStageFg][p];
     const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
-    for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)J.fgs)->lut[i >> 8][i & 255];
+    if (J.apply_fg)  // jobs without grain have no grain scratch and only copy
+        for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)J.fgs)->lut[i >> 8][i & 255];
     __syncthreads();
     const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
     const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
@@ -293,6 +294,50 @@ __global__ __launch_bounds__(kThreads) void k_fg_prep(const FrameJob *jobs, uint
     fg_prep_frame(J, sm);
 }
 
+// ---- side information built on the device from the walker's grids ----
+__constant__ uint8_t c_max_tx_rect[22];
+
+// get_tx_size for a chroma plane (cdf.rs:12976-13012): the largest
+// rectangular transform of the block's chroma residual size, 64-point
+// sides reduced to 32; None (no residual size) maps to TX_4X4 as the
+// loop-filter bridge does (cdf.rs:20588-20593).
+__device__ __forceinline__ uint32_t dev_chroma_tx(uint32_t mi_size, uint32_t sx, uint32_t sy) {
+    if (mi_size >= 22) return 0;
+    const uint32_t ps = c_subsampled[mi_size * 4 + sx * 2 + sy];
+    if (ps >= 22) return 0;
+    const uint32_t uv = c_max_tx_rect[ps];
+    const uint32_t w = c_tx_w[uv], h = c_tx_h[uv];
+    if (w == 64 || h == 64) return w == 16 ? 9u : (h == 16 ? 10u : 3u);  // 16x32, 32x16, 32x32
+    return uv;
+}
+
+// One thread per cell: the packed per-mi word and luma transform size of
+// every mi (loopfilter_tx_size_at, cdf.rs:20559-20593: inter cells take
+// InterTxSizes, intra cells TxSizes), then the chroma transform sizes of
+// the co-located luma mi of every chroma cell.
+__global__ void k_build_grids(oxav1_walker_grids g, uint32_t mi_rows, uint32_t mi_cols, uint32_t np, uint32_t sx,
+                              uint32_t sy, uint32_t *__restrict__ lf_mi, uint8_t *__restrict__ tx0,
+                              uint8_t *__restrict__ tx1, uint8_t *__restrict__ tx2) {
+    const uint32_t n0 = mi_rows * mi_cols;
+    const uint32_t crows = (mi_rows + sy) >> sy, ccols = (mi_cols + sx) >> sx, nc = crows * ccols;
+    const uint32_t n = n0 + (np > 1 ? 2 * nc : 0);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < n0) {
+            const uint32_t inter = g.is_inter[i] != 0;
+            lf_mi[i] = ((uint32_t)g.mi_size[i] << OXAV1_MI_SIZE_SHIFT) | ((uint32_t)(g.skip[i] != 0) << OXAV1_MI_SKIP_BIT) |
+                       ((inter ^ 1u) << OXAV1_MI_INTRA_BIT) | ((uint32_t)g.ref_frame[i] << OXAV1_MI_REF_SHIFT) |
+                       ((uint32_t)g.mode[i] << OXAV1_MI_MODE_SHIFT) | ((uint32_t)g.segment_id[i] << OXAV1_MI_SEG_SHIFT);
+            tx0[i] = inter ? g.inter_tx_size[i] : g.tx_size[i];
+        } else {
+            const uint32_t j = i - n0, plane = 1 + j / nc, k = j - (plane - 1) * nc;
+            const uint32_t r = k / ccols, c = k - r * ccols;
+            const uint32_t lr = r << sy, lc = c << sx;
+            const uint32_t v = lr < mi_rows && lc < mi_cols ? dev_chroma_tx(g.mi_size[lr * mi_cols + lc], sx, sy) : 0u;
+            (plane == 1 ? tx1 : tx2)[k] = (uint8_t)v;
+        }
+    }
+}
+
 // Zero the batched work counters (a kernel, so the reset is graph-safe).
 __global__ void k_reset(uint32_t *counters, uint32_t n) {
     if (threadIdx.x < n) counters[threadIdx.x] = 0;
@@ -326,6 +371,7 @@ static int ensure_device_init() {
     std::lock_guard<std::mutex> lk(g_init_mu);
     if (dev < 64 && g_init_done[dev]) return OXAV1_OK;
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_subsampled), kSubsampledSize, sizeof(kSubsampledSize)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_max_tx_rect), kMaxTxSizeRect, sizeof(kMaxTxSizeRect)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_upscale_filter), kUpscaleFilter, sizeof(kUpscaleFilter)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_params), kSgrParams, sizeof(kSgrParams)));
     uint16_t a2[256] = {0};
@@ -601,7 +647,15 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         }
     }
     // ---- loop restoration ----
-    J.lr_copy_none = 1;
+    // A job whose pre-grain frame is already final (no stage before the grain
+    // runs and the caller passes the stored frame as both src and ref_out:
+    // the show_existing_frame output, frame_driver.rs:1489-1537) skips the
+    // copy into ref_out.
+    bool in_place = !J.apply_lf && !J.apply_cdef && !J.use_sr && !J.uses_lr;
+    for (uint32_t p = 0; p < np; p++)
+        in_place = in_place && v.src[p] == v.ref_out[p] &&
+                   J.src_pitch[p] == (v.ref_pitch[p] ? v.ref_pitch[p] : J.out_cols[p]);
+    J.lr_copy_none = in_place ? 0 : 1;
     for (uint32_t p = 0; p < np; p++) {
         const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
         J.lr_type[p] = J.uses_lr ? d.lr.frame_restoration_type[p] : OXAV1_RESTORE_NONE;
@@ -668,7 +722,8 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         if (p < np) {
             const uint32_t psy = p ? sy : 0;
             J.lr_tiles_x[p] = ceil_div(J.out_cols[p], J.lr_tile_w[p]);
-            acc += lr_stripe_count(J.out_rows[p] - 1, psy) * lr_chunks(psy) * J.lr_tiles_x[p];
+            if (J.uses_lr || J.lr_copy_none)
+                acc += lr_stripe_count(J.out_rows[p] - 1, psy) * lr_chunks(psy) * J.lr_tiles_x[p];
         }
     }
     J.item_base[kStageLr][3] = acc;
@@ -1104,6 +1159,27 @@ extern "C" int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, u
     return OXAV1_OK;
 }
 
+extern "C" int oxav1_pf_build_grids(const oxav1_walker_grids *g, uint32_t mi_rows, uint32_t mi_cols,
+                                    uint8_t num_planes, uint8_t subsampling_x, uint8_t subsampling_y, uint32_t *lf_mi,
+                                    uint8_t *const lf_tx_size[3], void *hip_stream) {
+    if (!g || !lf_mi || !lf_tx_size || !lf_tx_size[0] || mi_rows == 0 || mi_cols == 0 ||
+        (num_planes != 1 && num_planes != 3))
+        return OXAV1_ERR_INVALID_ARG;
+    if (!g->is_inter || !g->skip || !g->ref_frame || !g->mode || !g->segment_id || !g->mi_size || !g->tx_size ||
+        !g->inter_tx_size || (num_planes > 1 && (!lf_tx_size[1] || !lf_tx_size[2])))
+        return OXAV1_ERR_INVALID_ARG;
+    int st = ensure_device_init();
+    if (st) return st;
+    const uint32_t sx = subsampling_x ? 1 : 0, sy = subsampling_y ? 1 : 0;
+    const uint64_t n = (uint64_t)mi_rows * mi_cols +
+                       (num_planes > 1 ? 2ull * ((mi_rows + sy) >> sy) * ((mi_cols + sx) >> sx) : 0);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_build_grids, dim3(blocks), dim3(256), 0, (hipStream_t)hip_stream, *g, mi_rows, mi_cols,
+                       (uint32_t)num_planes, sx, sy, lf_mi, lf_tx_size[0], lf_tx_size[1], lf_tx_size[2]);
+    HIP_TRY(hipGetLastError());
+    return OXAV1_OK;
+}
+
 // OXAV1_OK when all work enqueued on the engine's stream has finished,
 // OXAV1_ERR_BUSY while some is still running.  Does not block.
 extern "C" int oxav1_pf_engine_query(oxav1_pf_engine *e) {
@@ -1253,6 +1329,7 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_frame_dev") return sizeof(oxav1_frame_dev);
     if (n == "oxav1_server_config") return sizeof(oxav1_server_config);
     if (n == "oxav1_tuning") return sizeof(oxav1_tuning);
+    if (n == "oxav1_walker_grids") return sizeof(oxav1_walker_grids);
     if (n == "oxav1_server_done") return sizeof(oxav1_server_done);
     if (n == "oxav1_server_stats") return sizeof(oxav1_server_stats);
     return 0;

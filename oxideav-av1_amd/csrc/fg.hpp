@@ -360,20 +360,18 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
     const int32_t x = x0 + lane;
     const int32_t yb0 = y0 + 16 * wv, yb1 = min(y1, yb0 + 16);
     if (x >= x1 || yb0 >= yb1) return;
+    // A job without grain (apply_grain == 0, out != ref_out) has no grain
+    // scratch: it only copies.
     bool active;
-    const G<int16_t> *tmpl;
-    int tw;
-    if (plane == 0) {
+    if (!J.apply_fg) {
+        active = false;
+    } else if (plane == 0) {
         active = fg.num_y_points > 0;
-        tmpl = S->luma;
-        tw = kGrainW;
     } else {
         const bool any = fg.num_cb_points > 0 || fg.num_cr_points > 0 || fg.chroma_scaling_from_luma;
         active = J.np >= 3 && any &&
                  (plane == 1 ? (fg.num_cb_points > 0 || fg.chroma_scaling_from_luma)
                              : (fg.num_cr_points > 0 || fg.chroma_scaling_from_luma));
-        tmpl = plane == 1 ? S->cb : S->cr;
-        tw = S->chroma_w;
     }
     const G<T> *src = in + (size_t)yb0 * ip + x;
     G<T> *dst = out + (size_t)yb0 * op + x;
@@ -381,6 +379,8 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
         for (int32_t y = yb0; y < yb1; y++, src += ip, dst += op) *dst = *src;
         return;
     }
+    const G<int16_t> *tmpl = plane == 0 ? S->luma : (plane == 1 ? S->cb : S->cr);
+    const int tw = plane == 0 ? kGrainW : (int)S->chroma_w;
     // noise block / stripe geometry (film_grain.rs:626-651, 705-739)
     const uint32_t bsh = 5 - psx, ssh = 5 - psy;       // log2 block width / stripe height
     const uint32_t b = (uint32_t)x >> bsh;

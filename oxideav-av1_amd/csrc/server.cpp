@@ -287,6 +287,48 @@ extern "C" int oxav1_pf_server_submit(oxav1_pf_server *s, uint32_t session, cons
     return OXAV1_OK;
 }
 
+extern "C" int oxav1_pf_server_submit_existing(oxav1_pf_server *s, uint32_t session, uint32_t slot,
+                                               const oxav1_film_grain_params *fg, double arrival_us,
+                                               uint64_t *ticket) {
+    if (!s || !fg || session >= s->sessions.size() || !s->sessions[session].open) return OXAV1_ERR_INVALID_ARG;
+    Session &ss = s->sessions[session];
+    if (slot >= ss.nslots || !ss.grain) return OXAV1_ERR_INVALID_ARG;
+    if (ss.busy[slot]) return OXAV1_ERR_BUSY;
+    // Nothing before the grain stage runs; the stored frame is both the
+    // source and the pre-grain reference, the slot's grain buffer the output.
+    oxav1_frame_desc d = ss.shape;
+    d.apply_lf = 0;
+    d.apply_cdef = 0;
+    d.use_superres = 0;
+    d.frame_width = std::max(d.upscaled_width, d.frame_width);
+    d.upscaled_width = d.frame_width;
+    d.lr.uses_lr = 0;
+    d.fg = *fg;
+    d.mi_rows = (d.frame_height + 3) / 4;  // the mi grid of the stored (upscaled) frame
+    d.mi_cols = (d.frame_width + 3) / 4;
+    Pending f;
+    f.ticket = s->next_ticket++;
+    f.session = session;
+    f.slot = slot;
+    f.desc = d;
+    std::memset(&f.dev, 0, sizeof f.dev);
+    for (uint32_t p = 0; p < ss.shape.num_planes; p++) {
+        f.dev.src[p] = ss.ref[slot * 3 + p];
+        f.dev.src_pitch[p] = ss.cols[p];
+        f.dev.ref_out[p] = ss.ref[slot * 3 + p];
+        f.dev.ref_pitch[p] = ss.cols[p];
+        f.dev.out[p] = ss.out[slot * 3 + p];
+        f.dev.out_pitch[p] = ss.cols[p];
+    }
+    f.arrival = arrival_us;
+    ss.busy[slot] = 1;
+    ss.pending++;
+    s->ready.push_back(f);
+    s->submitted++;
+    if (ticket) *ticket = f.ticket;
+    return OXAV1_OK;
+}
+
 static void retire(oxav1_pf_server *s, Lane &l, int status, double t) {
     const uint32_t n = (uint32_t)l.frames.size();
     for (const Pending &f : l.frames) {

@@ -156,6 +156,11 @@ class GenFrame(C.Structure):
     ]
 
 
+class WalkerGrids(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("is_inter", "skip", "ref_frame", "mode", "segment_id", "mi_size",
+                                           "tx_size", "inter_tx_size")]
+
+
 class Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", (C.c_uint16 * 2) * 5), ("waves_per_eu", (u8 * 2) * 5),
                 ("sr_rows_per_item", u8 * 2), ("lr_tile_w", u8 * 2), ("sub_batches", u8 * 2),
@@ -187,7 +192,7 @@ ABI_STRUCTS = {
     "oxav1_sr_ctx": SrCtx, "oxav1_lr_unit": LrUnit, "oxav1_lr_ctx": LrCtx,
     "oxav1_lr_grids": LrGrids, "oxav1_film_grain_params": FilmGrainParams,
     "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev,
-    "oxav1_tuning": Tuning, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
+    "oxav1_tuning": Tuning, "oxav1_walker_grids": WalkerGrids, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
     "oxav1_server_stats": ServerStats,
 }
 
@@ -217,6 +222,8 @@ _SIGS = {
     "oxav1_pf_engine_stage_ms": (C.c_int, [C.c_void_p, C.c_char_p, _P(C.c_double), _P(u32)]),
     "oxav1_pf_engine_stream": (C.c_void_p, [C.c_void_p]),
     "oxav1_pf_engine_query": (C.c_int, [C.c_void_p]),
+    "oxav1_pf_build_grids": (C.c_int, [_P(WalkerGrids), u32, u32, u8, u8, u8, C.c_void_p, C.c_void_p * 3,
+                                       C.c_void_p]),
     "oxav1_pf_engine_set_tuning": (C.c_int, [C.c_void_p, _P(Tuning)]),
     "oxav1_pf_engine_get_tuning": (C.c_int, [C.c_void_p, _P(Tuning), _P(u32), _P((u32 * 4) * 5)]),
     "oxav1_pf_engine_reserve": (C.c_int, [C.c_void_p, _P(FrameDesc), u32, _P(u64)]),
@@ -229,6 +236,8 @@ _SIGS = {
     "oxav1_pf_server_slot": (C.c_int, [C.c_void_p, u32, u32, u32, C.c_int, _P(C.c_void_p), _P(u32)]),
     "oxav1_pf_server_submit": (
         C.c_int, [C.c_void_p, u32, _P(FrameDesc), _P(FrameDev), u32, C.c_double, _P(u64)]),
+    "oxav1_pf_server_submit_existing": (
+        C.c_int, [C.c_void_p, u32, u32, _P(FilmGrainParams), C.c_double, _P(u64)]),
     "oxav1_pf_server_poll": (C.c_int, [C.c_void_p, _P(ServerDone), u32, _P(u32)]),
     "oxav1_pf_server_stats": (C.c_int, [C.c_void_p, _P(ServerStats)]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),

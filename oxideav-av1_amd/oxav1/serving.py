@@ -132,6 +132,16 @@ class Server:
         abi.check(st, "server_submit")
         return t.value
 
+    def submit_existing(self, sid, slot, fg, arrival_us):
+        """show_existing_frame output of a slot with grain parameters fg;
+        ticket, or None when the slot is still busy."""
+        t = C.c_uint64()
+        st = self.lib.oxav1_pf_server_submit_existing(self.h, sid, slot, C.byref(fg), arrival_us, C.byref(t))
+        if st == abi.ERR_BUSY:
+            return None
+        abi.check(st, "server_submit_existing")
+        return t.value
+
     def poll(self):
         """Finished frames as a list of ServerDone copies."""
         out = []

@@ -107,3 +107,60 @@ def test_serving_slot_busy_until_done():
         assert srv.close_session(s.sid)
     finally:
         srv.close()
+
+
+def _drive(srv, n_expected, timeout_us=30e6):
+    done, t0 = [], srv.now_us()
+    while len(done) < n_expected:
+        done += srv.poll()
+        assert srv.now_us() - t0 < timeout_us, "frames did not finish"
+    return done
+
+
+def test_serving_show_existing_frame(oracle_mod):
+    """show_existing_frame output (frame_driver.rs:1489-1537): grain from a
+    slot's saved parameters applied to the stored pre-grain frame, into the
+    slot's grain buffer; equals the oracle's film_grain_synthesis of the
+    stored frame, and the stored frame stays as it was.  Also the no-grain
+    case (a copy)."""
+    from oxav1 import abi
+    from oxav1.serving import Server, expected_outputs
+    sessions = _sessions([MIX[1], MIX[3]], n_frames=2, fps=30.0, seed0=0x5E0E)
+    expected = expected_outputs(sessions)
+    calls = oracle_mod.calls()
+    srv = Server(lanes=2, max_batch=4)
+    try:
+        for s in sessions:
+            assert s.admit(srv)
+            for k in range(2):
+                desc, dev, slot = s.frame_job(k)
+                assert srv.submit(s.sid, desc, dev, slot, srv.now_us()) is not None
+        assert all(r.status == 0 for r in _drive(srv, 4))
+        for i, s in enumerate(sessions):
+            d = s.desc
+            fg = abi.FilmGrainParams.from_buffer_copy(d.fg)
+            fg.grain_seed = (fg.grain_seed * 7 + 1234) & 0xFFFF  # the slot's saved parameters, reseeded
+            for slot in range(2):
+                assert srv.submit_existing(s.sid, slot, fg, srv.now_us()) is not None
+            assert srv.submit_existing(s.sid, 0, fg, srv.now_us()) is None  # busy until done
+            assert all(r.status == 0 for r in _drive(srv, 2))
+            for slot in range(2):
+                ref_e, _ = expected[(i, slot)]
+                want = calls.film_grain_synthesis(fg, d.bit_depth, d.num_planes, d.subsampling_x,
+                                                  d.subsampling_y, d.matrix_coefficients, ref_e)
+                for p in range(d.num_planes):
+                    stored = srv.slot(s.sid, slot, p, 0, s.out_shape(p), s.sample_dtype).astype(np.int32)
+                    shown = srv.slot(s.sid, slot, p, 1, s.out_shape(p), s.sample_dtype).astype(np.int32)
+                    assert np.array_equal(stored, ref_e[p]), (i, slot, p, "stored frame changed")
+                    assert np.array_equal(shown, want[p]), (i, slot, p, "shown frame != oracle grain")
+            off = abi.FilmGrainParams.from_buffer_copy(fg)
+            off.apply_grain = 0
+            assert srv.submit_existing(s.sid, 1, off, srv.now_us()) is not None
+            _drive(srv, 1)
+            for p in range(d.num_planes):
+                shown = srv.slot(s.sid, 1, p, 1, s.out_shape(p), s.sample_dtype).astype(np.int32)
+                assert np.array_equal(shown, expected[(i, 1)][0][p]), (i, p, "no-grain output != stored")
+        for s in sessions:
+            assert srv.close_session(s.sid)
+    finally:
+        srv.close()
