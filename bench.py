@@ -188,6 +188,71 @@ def serve(args):
     }
 
 
+def side_paths(reps=20):
+    """Latency of the two side paths on the metric's frame shape."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oxav1 import Frame, abi
+    from oxav1.engine import DevBuffer
+    from oxav1.serving import DecodeSession, Server
+
+    lib = abi.load()
+    f = Frame.generate(2, 0x0A1F0002)
+    d = f.desc
+    # walker grids -> packed words and transform sizes
+    walker = dict(is_inter=(f.is_intra == 0), skip=f.skip, ref_frame=f.ref_frame, mode=f.mode,
+                  segment_id=f.segment_id, mi_size=f.mi_size, tx_size=f.lf_tx_size[0],
+                  inter_tx_size=f.lf_tx_size[0])
+    keep = {k: DevBuffer(v.shape, np.uint8, np.ascontiguousarray(v, dtype=np.uint8)) for k, v in walker.items()}
+    g = abi.WalkerGrids(**{k: b.ptr for k, b in keep.items()})
+    lf_mi = DevBuffer((d.mi_rows, d.mi_cols), np.uint32)
+    tx = [DevBuffer(f.lf_tx_size[p].shape, np.uint8) for p in range(3)]
+    ptrs = (C.c_void_p * 3)(*[t.ptr for t in tx])
+
+    def build():
+        abi.check(lib.oxav1_pf_build_grids(C.byref(g), d.mi_rows, d.mi_cols, d.num_planes, d.subsampling_x,
+                                           d.subsampling_y, lf_mi.ptr, ptrs, None), "build_grids")
+    build()
+    exact = np.array_equal(lf_mi.download(), f.lf_mi) and all(
+        np.array_equal(tx[p].download(), f.lf_tx_size[p]) for p in range(3))
+    t = time.perf_counter()
+    for _ in range(reps):
+        build()
+    lf_mi.download()  # synchronises the default stream
+    grid_ms = (time.perf_counter() - t) / reps * 1e3
+    for b in list(keep.values()) + [lf_mi] + tx:
+        b.free()
+    # show_existing_frame on a resident slot
+    cfg = abi.GenConfig()
+    abi.check(lib.oxav1_gen_default_config(2, 0x0A1F0002, C.byref(cfg)), "config")
+    sess = DecodeSession(0x0A1F0002, n_frames=1, pool=1, slots=1, gen_config=cfg)
+    srv = Server(lanes=1, max_batch=1)
+    try:
+        assert sess.admit(srv)
+        desc, dev, slot = sess.frame_job(0)
+        srv.submit(sess.sid, desc, dev, slot, srv.now_us())
+        while not srv.poll():
+            pass
+        lat = []
+        for _ in range(reps):
+            t0 = srv.now_us()
+            srv.submit_existing(sess.sid, 0, sess.desc.fg, t0)
+            done = []
+            while not done:
+                done = srv.poll()
+            lat.append((done[0].done_us - t0) * 1e-3)
+    finally:
+        srv.close()
+    return {"what": "side paths on one 3840x2160 10-bit 4:2:0 frame",
+            "build_grids_ms": round(grid_ms, 4), "build_grids_exact": bool(exact),
+            "mi_cells": int(d.mi_rows * d.mi_cols),
+            "show_existing_ms": {"p50": round(statistics.median(lat), 3), "max": round(max(lat), 3)},
+            "show_existing_what": "film grain from the slot's parameters on the resident pre-grain frame, "
+                                  "submit to completion seen by the serving loop"}
+
+
 def describe_sessions(sessions):
     from collections import Counter
     c = Counter(f"{s.desc.upscaled_width}x{s.desc.frame_height} {s.desc.bit_depth}-bit" for s in sessions)
@@ -215,6 +280,9 @@ def main():
     ap.add_argument("--session-frames", type=int, default=60, help="frames each session decodes")
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=16)
+    ap.add_argument("--side", action="store_true",
+                    help="time the side paths on one 3840x2160 10-bit frame: side information built on the "
+                         "GPU from walker grids, and the show_existing_frame grain output of a resident slot")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run only the plain device copy kernel over this many bytes, --steps times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters against a known traffic)")
@@ -254,6 +322,9 @@ def main():
 
     if args.serve:
         print(json.dumps(serve(args)))
+        return
+    if args.side:
+        print(json.dumps(side_paths()))
         return
 
     seeds = workload_seeds(args.cfg, args.frames)

@@ -287,7 +287,7 @@ OXAV1_STAGE_KERNELS(fg)
 
 // One workgroup per frame; frames without film grain return at once.
 __global__ __launch_bounds__(kThreads) void k_fg_prep(const FrameJob *jobs, uint32_t njobs) {
-    __shared__ int16_t sm[3 * kGrainH * kGrainW];
+    __shared__ int16_t sm[4 * kGrainH * kGrainW];  // luma, Cb, Cr templates and the chroma luma term
     if (blockIdx.x >= njobs) return;
     const FrameJob &J = jobs[blockIdx.x];
     if (!J.apply_fg) return;

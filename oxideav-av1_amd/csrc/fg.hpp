@@ -56,26 +56,22 @@ __device__ __forceinline__ uint32_t fg_jump(uint32_t r, uint32_t n) {
 
 __device__ __forceinline__ int32_t fg_round2(int32_t x, uint32_t n) { return n == 0 ? x : (x + (1 << (n - 1))) >> n; }
 
-// scaling_lookup_fill (film_grain.rs:480-526), one lane per plane
-__device__ void fg_lut_fill(G<uint8_t> *t, uint32_t np_, const uint8_t *xs, const uint8_t *ys) {
-    if (np_ == 0) {
-        for (int v = 0; v < 256; v++) t[v] = 0;
-        return;
-    }
-    const uint32_t x0 = xs[0];
-    for (uint32_t v = 0; v < min(x0, 256u); v++) t[v] = ys[0];
+// scaling_lookup_fill (film_grain.rs:480-526) for one table entry v: the
+// left plateau, then the ramps in segment order (a later segment wins where
+// non-increasing points make ramps overlap), then the right plateau;
+// entries nothing covers stay 0, as in the reference's fresh table.
+__device__ __forceinline__ uint8_t fg_lut_value(uint32_t np_, const uint8_t *xs, const uint8_t *ys, int32_t v) {
+    if (np_ == 0) return 0;
+    int32_t val = v < (int32_t)xs[0] ? (int32_t)ys[0] : 0;
     for (uint32_t i = 0; i + 1 < np_; i++) {
         const int32_t xa = xs[i], xb = xs[i + 1], ya = ys[i], yb = ys[i + 1];
         const int32_t dx = xb - xa;
-        if (dx <= 0) continue;
+        if (dx <= 0 || v < xa || v >= xb) continue;
         const int32_t delta = (yb - ya) * ((65536 + (dx >> 1)) / dx);
-        for (int32_t x = 0; x < dx; x++) {
-            const int32_t idx = xa + x;
-            if (idx < 256) t[idx] = (uint8_t)clip3(0, 255, ya + ((x * delta + 32768) >> 16));
-        }
+        val = clip3(0, 255, ya + (((v - xa) * delta + 32768) >> 16));
     }
-    const uint32_t xn = xs[np_ - 1];
-    for (uint32_t v = min(xn, 256u); v < 256; v++) t[v] = ys[np_ - 1];
+    if (v >= (int32_t)xs[np_ - 1]) val = ys[np_ - 1];
+    return (uint8_t)val;
 }
 
 // White noise for one template (generate_grain white-noise loops).
@@ -95,77 +91,90 @@ __device__ void fg_white(int16_t *g, int count, uint32_t seed, bool on, uint32_t
 }
 
 // Causal auto-regressive filter of NT templates of width w, height h in LDS
-// (film_grain.rs:255-289, 319-372).  Wavefront: lane j filters row 3+j and at
-// step t reaches column 3+t-4j, so row y-1 finished column x+3 one step
-// earlier.  Each lane slides the LAG rows above (columns x-LAG..x+LAG) and
-// the last LAG outputs of its own row through registers; per step it reads
-// one new column of the rows above.  Coefficient order follows the
-// reference's pos counter: rows -LAG..-1 left to right, then row 0 columns
-// -LAG..-1, then (for chroma) the luma-average tap.
+// (film_grain.rs:255-289, 319-372), run by one wave alone.  Wavefront:
+// lane j filters row r0+j and at step t reaches column 3+t-4j, so row y-1
+// finished column x+3 one step earlier.  Within one wave LDS accesses are
+// performed in program order, so no workgroup barrier is needed between
+// steps (a compiler-only fence keeps the accesses in order); templates
+// taller than 64 filtered rows take a second pass over the remaining rows,
+// which start after every row of the first pass is done.  Each lane slides
+// the LAG rows above (columns x-LAG..x+LAG) and the last LAG outputs of its
+// own row through registers; per step it reads one new column of the rows
+// above.  Coefficient order follows the reference's pos counter: rows
+// -LAG..-1 left to right, then row 0 columns -LAG..-1, then (for chroma) the
+// luma-average tap.
 template <int LAG, int NT, typename LumaTerm>
 __device__ void fg_ar(int16_t *const *g, int w, int h, const int32_t (*cf)[25], uint32_t ash, int32_t gmin,
-                      int32_t gmax, LumaTerm luma) {
+                      int32_t gmax, LumaTerm luma, int wave = 0) {
     constexpr int K = 2 * LAG + 1;
     constexpr int R = LAG > 0 ? LAG : 1;
-    const int rows = h - 3, cols = w - 6;
-    const int steps = cols + 4 * (rows - 1);
-    const int j = threadIdx.x;
-    const int y = 3 + j;
-    int32_t win[NT][R][K];
-    int32_t cur[NT][R];
-    for (int t = 0; t < steps; t++) {
-        const int x = 3 + t - 4 * j;
-        if (j < rows && x >= 3 && x < w - 3) {
-            if constexpr (LAG > 0) {
-                if (x == 3) {
+    if ((int)(threadIdx.x >> 6) != wave) return;
+    const int rows_all = h - 3, cols = w - 6;
+    const int j = threadIdx.x & 63;
+    for (int r0 = 0; r0 < rows_all; r0 += 64) {
+        const int rows = min(64, rows_all - r0);
+        const int steps = cols + 4 * (rows - 1);
+        const int y = 3 + r0 + j;
+        int32_t win[NT][R][K];
+        int32_t cur[NT][R];
+        for (int t = 0; t < steps; t++) {
+            const int x = 3 + t - 4 * j;
+            if (j < rows && x >= 3 && x < w - 3) {
+                if constexpr (LAG > 0) {
+                    if (x == 3) {
 #pragma unroll
-                    for (int n = 0; n < NT; n++) {
+                        for (int n = 0; n < NT; n++) {
+#pragma unroll
+                            for (int r = 0; r < LAG; r++)
+#pragma unroll
+                                for (int k = 0; k < K; k++) win[n][r][k] = g[n][(y - LAG + r) * w + (x - LAG + k)];
+#pragma unroll
+                            for (int k = 0; k < LAG; k++) cur[n][k] = g[n][y * w + (x - LAG + k)];
+                        }
+                    } else {
+#pragma unroll
+                        for (int n = 0; n < NT; n++)
+#pragma unroll
+                            for (int r = 0; r < LAG; r++) {
+#pragma unroll
+                                for (int k = 0; k < K - 1; k++) win[n][r][k] = win[n][r][k + 1];
+                                win[n][r][K - 1] = g[n][(y - LAG + r) * w + (x + LAG)];
+                            }
+                    }
+                }
+                const int32_t lt = luma(0, x, y);
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    // four partial sums keep the multiply-add chain short
+                    int32_t s4[4] = {0, 0, 0, 0};
+                    if constexpr (LAG > 0) {
 #pragma unroll
                         for (int r = 0; r < LAG; r++)
 #pragma unroll
-                            for (int k = 0; k < K; k++) win[n][r][k] = g[n][(y - LAG + r) * w + (x - LAG + k)];
+                            for (int k = 0; k < K; k++) s4[(r * K + k) & 3] += win[n][r][k] * cf[n][r * K + k];
 #pragma unroll
-                        for (int k = 0; k < LAG; k++) cur[n][k] = g[n][y * w + (x - LAG + k)];
+                        for (int k = 0; k < LAG; k++) s4[(LAG * K + k) & 3] += cur[n][k] * cf[n][LAG * K + k];
                     }
-                } else {
+                    const int32_t s = s4[0] + s4[1] + s4[2] + s4[3] + lt * cf[n][LAG * K + LAG];
+                    const int32_t v = clip3(gmin, gmax, (int32_t)g[n][y * w + x] + fg_round2(s, ash));
+                    g[n][y * w + x] = (int16_t)v;
+                    if constexpr (LAG > 0) {
 #pragma unroll
-                    for (int n = 0; n < NT; n++)
-#pragma unroll
-                        for (int r = 0; r < LAG; r++) {
-#pragma unroll
-                            for (int k = 0; k < K - 1; k++) win[n][r][k] = win[n][r][k + 1];
-                            win[n][r][K - 1] = g[n][(y - LAG + r) * w + (x + LAG)];
-                        }
+                        for (int k = 0; k < LAG - 1; k++) cur[n][k] = cur[n][k + 1];
+                        cur[n][LAG - 1] = v;
+                    }
                 }
             }
-            const int32_t lt = luma(0, x, y);
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-                int32_t s = 0;
-                if constexpr (LAG > 0) {
-#pragma unroll
-                    for (int r = 0; r < LAG; r++)
-#pragma unroll
-                        for (int k = 0; k < K; k++) s += win[n][r][k] * cf[n][r * K + k];
-#pragma unroll
-                    for (int k = 0; k < LAG; k++) s += cur[n][k] * cf[n][LAG * K + k];
-                }
-                s += lt * cf[n][LAG * K + LAG];
-                const int32_t v = clip3(gmin, gmax, (int32_t)g[n][y * w + x] + fg_round2(s, ash));
-                g[n][y * w + x] = (int16_t)v;
-                if constexpr (LAG > 0) {
-#pragma unroll
-                    for (int k = 0; k < LAG - 1; k++) cur[n][k] = cur[n][k + 1];
-                    cur[n][LAG - 1] = v;
-                }
-            }
+            // next step's reads of the rows above must follow this step's
+            // writes: keep the compiler from moving LDS accesses across
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_wave_barrier();
         }
-        __syncthreads();
     }
 }
 
 // One workgroup per frame (blockDim.x >= 128).
-__device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) {
+__device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 4 * 73 * 82 */) {
     G<FgScratch> *S = (G<FgScratch> *)J.fgs;
     const oxav1_film_grain_params &fg = J.fg;
     const int tid = threadIdx.x;
@@ -176,26 +185,35 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
     const int cw = sx ? 44 : 82, ch = sy ? 38 : 73;
     int16_t *L = sm, *Cb = sm + kGrainH * kGrainW, *Cr = Cb + kGrainH * kGrainW;
     const uint32_t np_ = J.np < 3 ? J.np : 3;
-    // scaling LUTs (scaling_lookup_init, film_grain.rs:418-466)
-    if (tid < 3) {
-        G<uint8_t> *t = S->lut[tid];
-        if ((uint32_t)tid >= np_)
-            for (int v = 0; v < 256; v++) t[v] = 0;
-        else if (tid == 0 || fg.chroma_scaling_from_luma)
-            fg_lut_fill(t, fg.num_y_points, fg.point_y_value, fg.point_y_scaling);
-        else if (tid == 1)
-            fg_lut_fill(t, fg.num_cb_points, fg.point_cb_value, fg.point_cb_scaling);
-        else
-            fg_lut_fill(t, fg.num_cr_points, fg.point_cr_value, fg.point_cr_scaling);
+    // scaling LUTs (scaling_lookup_init, film_grain.rs:418-466): a thread per
+    // entry of each table
+    for (int i = tid; i < 3 * 256; i += blockDim.x) {
+        const int p = i >> 8, v = i & 255;
+        uint8_t val = 0;
+        if ((uint32_t)p < np_) {
+            if (p == 0 || fg.chroma_scaling_from_luma)
+                val = fg_lut_value(fg.num_y_points, fg.point_y_value, fg.point_y_scaling, v);
+            else if (p == 1)
+                val = fg_lut_value(fg.num_cb_points, fg.point_cb_value, fg.point_cb_scaling, v);
+            else
+                val = fg_lut_value(fg.num_cr_points, fg.point_cr_value, fg.point_cr_scaling, v);
+        }
+        S->lut[p][v] = val;
     }
-    // per-stripe block offsets (film_grain.rs:626-641)
-    for (uint32_t s = tid; s < S->nstripes; s += blockDim.x) {
-        uint32_t r = fg.grain_seed;
-        r ^= ((s * 37u + 178u) & 255u) << 8;
-        r ^= (s * 173u + 105u) & 255u;
-        for (uint32_t b = 0; b < S->nblocks; b++) {
-            fg_step(r);
-            ((G<uint8_t> *)S->offsets)[s * S->nblocks + b] = (uint8_t)((r >> 8) & 255u);
+    // per-stripe block offsets (film_grain.rs:626-641): the stripe's LFSR
+    // draws split into runs of 16, each run reached by a jump ahead
+    {
+        const uint32_t nst = S->nstripes, nbl = S->nblocks, nrun = (nbl + 15) / 16;
+        for (uint32_t w = tid; w < nst * nrun; w += blockDim.x) {
+            const uint32_t s = w / nrun, b0 = (w - s * nrun) * 16;
+            uint32_t r = fg.grain_seed;
+            r ^= ((s * 37u + 178u) & 255u) << 8;
+            r ^= (s * 173u + 105u) & 255u;
+            r = fg_jump(r & 0xffffu, b0);
+            for (uint32_t b = b0; b < min(b0 + 16, nbl); b++) {
+                fg_step(r);
+                ((G<uint8_t> *)S->offsets)[s * nbl + b] = (uint8_t)((r >> 8) & 255u);
+            }
         }
     }
     if (tid == 0) {
@@ -226,30 +244,49 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 3 * 73 * 82 */) 
         }
     }
     __syncthreads();
-    // chroma AR, Cb and Cr together (film_grain.rs:319-372); the centre tap
-    // adds the co-located luma average when num_y_points > 0
+    // chroma AR (film_grain.rs:319-372): Cb on the first wave and Cr on the
+    // second, concurrently; the centre tap adds the co-located luma average
+    // (when num_y_points > 0), precomputed for every chroma position
     if (fg.chroma_scaling_from_luma || fg.num_cb_points > 0 || fg.num_cr_points > 0) {
-        int16_t *gs[2] = {Cb, Cr};
-        int32_t cf[2][25];
-        for (int i = 0; i < 25; i++) {
-            cf[0][i] = (int32_t)fg.ar_coeffs_cb_plus_128[i] - 128;
-            cf[1][i] = (int32_t)fg.ar_coeffs_cr_plus_128[i] - 128;
-        }
         const bool with_luma = fg.num_y_points > 0;
+        int16_t *LT = Cr + kGrainH * kGrainW;  // luma term per chroma position
         const int isx = (int)sx, isy = (int)sy;
-        auto luma = [=](int n, int x, int y) -> int32_t {
-            if (!with_luma) return 0;
-            const int lx = ((x - 3) << isx) + 3, ly = ((y - 3) << isy) + 3;
+        for (int i = tid; i < cw * ch; i += blockDim.x) {
+            const int y = i / cw, x = i - y * cw;
             int32_t l = 0;
-            for (int i = 0; i <= isy; i++)
-                for (int k = 0; k <= isx; k++) l += (int32_t)L[(ly + i) * kGrainW + (lx + k)];
-            return fg_round2(l, (uint32_t)(isx + isy));
-        };
+            if (with_luma && y >= 3 && x >= 3 && x < cw - 3) {
+                const int lx = ((x - 3) << isx) + 3, ly = ((y - 3) << isy) + 3;
+                for (int a = 0; a <= isy; a++)
+                    for (int k = 0; k <= isx; k++) l += (int32_t)L[(ly + a) * kGrainW + (lx + k)];
+                l = fg_round2(l, (uint32_t)(isx + isy));
+            }
+            LT[i] = (int16_t)l;
+        }
+        __syncthreads();
+        int32_t cfb[1][25], cfr[1][25];
+        for (int i = 0; i < 25; i++) {
+            cfb[0][i] = (int32_t)fg.ar_coeffs_cb_plus_128[i] - 128;
+            cfr[0][i] = (int32_t)fg.ar_coeffs_cr_plus_128[i] - 128;
+        }
+        auto luma = [=](int, int x, int y) -> int32_t { return (int32_t)LT[y * cw + x]; };
+        int16_t *gb[1] = {Cb}, *gr[1] = {Cr};
         switch (lag) {
-        case 0: fg_ar<0, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
-        case 1: fg_ar<1, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
-        case 2: fg_ar<2, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
-        default: fg_ar<3, 2>(gs, cw, ch, cf, ash, gmin, gmax, luma); break;
+        case 0:
+            fg_ar<0, 1>(gb, cw, ch, cfb, ash, gmin, gmax, luma, 0);
+            fg_ar<0, 1>(gr, cw, ch, cfr, ash, gmin, gmax, luma, 1);
+            break;
+        case 1:
+            fg_ar<1, 1>(gb, cw, ch, cfb, ash, gmin, gmax, luma, 0);
+            fg_ar<1, 1>(gr, cw, ch, cfr, ash, gmin, gmax, luma, 1);
+            break;
+        case 2:
+            fg_ar<2, 1>(gb, cw, ch, cfb, ash, gmin, gmax, luma, 0);
+            fg_ar<2, 1>(gr, cw, ch, cfr, ash, gmin, gmax, luma, 1);
+            break;
+        default:
+            fg_ar<3, 1>(gb, cw, ch, cfb, ash, gmin, gmax, luma, 0);
+            fg_ar<3, 1>(gr, cw, ch, cfr, ash, gmin, gmax, luma, 1);
+            break;
         }
     }
     __syncthreads();

@@ -161,6 +161,23 @@ def test_film_grain_chroma_from_luma(gpu_calls, oracle_mod):
         oracle_mod.calls().film_grain_synthesis(*args, planes), "fg chroma_scaling_from_luma")
 
 
+def test_film_grain_scaling_points_out_of_order(gpu_calls, oracle_mod):
+    """Scaling points that are not increasing (a stream the conformance rules
+    forbid) build the tables the reference builds (film_grain.rs:480-526):
+    ramps in segment order, skipped segments, later ramps winning."""
+    f = _frame(0x5A22, enable_fg=1, fg_overlap=0, fg_ar_lag=1, fg_num_y_points=6, fg_chroma_points=4,
+               upscaled_width=120, frame_height=66)
+    fg = f.desc.fg
+    xs, ys = [20, 90, 60, 60, 200, 150], [40, 200, 10, 90, 255, 30]
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        fg.point_y_value[i], fg.point_y_scaling[i] = x, y
+    fg.point_cb_value[0], fg.point_cb_value[1], fg.point_cb_value[2] = 100, 50, 120
+    planes = [f.planes[p][: f.out_shape(p)[0], : f.out_shape(p)[1]].copy() for p in range(3)]
+    args = (fg, 8, 3, 1, 1, 1)
+    _eq(gpu_calls.film_grain_synthesis(*args, planes),
+        oracle_mod.calls().film_grain_synthesis(*args, planes), "fg scaling points out of order")
+
+
 @pytest.mark.parametrize("case", [
     dict(cfg=0),
     dict(kw=dict(bit_depth=10, noise_amp=32, superres_denom=12, upscaled_width=200, frame_height=96,
