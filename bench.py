@@ -253,6 +253,87 @@ def side_paths(reps=20):
                                   "submit to completion seen by the serving loop"}
 
 
+def elect_bench(reps=5, cpu=True):
+    """The encoder's CDEF strength election (encoder/cdef_elect.rs) on one
+    frame: the whole election call (host planes in, plan out), one launch of
+    the per-unit SSD kernel over the first sweep's candidates on resident
+    planes, and the C oracle's election on the 1080p frame for comparison."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oxav1 import abi
+    from oxav1.elect import elect_cdef, synthetic_input
+    from oxav1.engine import DevBuffer
+
+    lib = abi.load()
+    out = []
+    for w, h, bd in ((1920, 1080, 10), (3840, 2160, 10)):
+        inp = synthetic_input(0xE1EC7 + w, w, h, bd, 1, 1, lambda_=600 * 16, max_bits=3)
+        r = elect_cdef(inp)
+        lat = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            elect_cdef(inp)
+            lat.append((time.perf_counter() - t) * 1e3)
+        # the SSD kernel alone: the zero rows and the coarse and
+        # secondary-only grids of both dampings and plane sets (78 candidates)
+        cands = [abi.CdefCand(3, 0, 0, 0), abi.CdefCand(3, 1, 0, 0)]
+        for d in (3, 5):
+            for ps in (0, 1):
+                cands += [abi.CdefCand(d, ps, p, s) for p in (1, 2, 3, 4, 6, 9, 12, 15) for s in (0, 2)]
+                cands += [abi.CdefCand(d, ps, 0, s) for s in (1, 2, 4)]
+        f = abi.CdefSsdFrame()
+        f.mi_rows, f.mi_cols = inp.mi_rows, inp.mi_cols
+        keep = [DevBuffer(inp.cdef_idx.shape, np.int8, inp.cdef_idx), DevBuffer(inp.skip.shape, np.uint8, inp.skip)]
+        f.cdef_idx, f.skip = keep[0].ptr, keep[1].ptr
+        for p in range(3):
+            keep += [DevBuffer(inp.recon[p].shape, np.uint16, inp.recon[p]),
+                     DevBuffer(inp.input[p].shape, np.uint16, inp.input[p])]
+            f.recon[p], f.input[p] = keep[-2].ptr, keep[-1].ptr
+            f.rows[p], f.cols[p] = inp.recon[p].shape
+            f.recon_pitch[p] = f.input_pitch[p] = inp.recon[p].shape[1]
+        f.bit_depth, f.subsampling_x, f.subsampling_y, f.num_planes, f.sample_bytes = bd, 1, 1, 3, 2
+        ssd = DevBuffer((len(cands), inp.n_units), np.uint64)
+        keep.append(ssd)
+        f.ssd = ssd.ptr
+        ca = (abi.CdefCand * len(cands))(*cands)
+        abi.check(lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, ca, len(cands), None), "cdef_unit_ssd")
+        kl = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            abi.check(lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, ca, len(cands), None), "cdef_unit_ssd")
+            kl.append((time.perf_counter() - t) * 1e3)
+        for b in keep:
+            b.free()
+        luma, chroma = w * h, 2 * ((w + 1) // 2) * ((h + 1) // 2)
+        samples = sum(luma if c.plane_set == 0 else chroma for c in cands)
+        k_ms = statistics.median(kl)
+        row = {"frame": f"{w}x{h} {bd}-bit 4:2:0", "units": inp.n_units,
+               "elect_ms": {"p50": round(statistics.median(lat), 2), "min": round(min(lat), 2)},
+               "ssd_launch_ms": round(k_ms, 3), "ssd_launch_candidates": len(cands),
+               "filtered_gsamples_per_s": round(samples / (k_ms * 1e-3) / 1e9, 1),
+               "plan": None if r is None else {"cdef_bits": r.best.params["cdef_bits"],
+                                               "damping": r.best.params["cdef_damping"],
+                                               "d": r.best.d, "base_d": r.base_d}}
+        if cpu and w == 1920:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            t = time.perf_counter()
+            want = elect_cdef(inp, oracle.load(), "oo_cdef_elect")
+            cpu_s = time.perf_counter() - t
+            same = (want is None and r is None) or (
+                want is not None and r is not None and want.best.params == r.best.params and want.best.d == r.best.d
+                and np.array_equal(want.best.unit_idx, r.best.unit_idx) and want.base_d == r.base_d)
+            row["cpu_oracle"] = {"s": round(cpu_s, 2), "cores": 1, "kind": "port",
+                                 "speedup": round(cpu_s * 1e3 / statistics.median(lat), 1)}
+            row["byte_exact_vs_oracle"] = bool(same)
+        out.append(row)
+    return {"what": "encoder CDEF strength election (sweeps, damping refinement, greedy per-unit sets, "
+                    "D*256 + lambda*R256), host u16 planes in, elected plan out",
+            "frames": out}
+
+
 def describe_sessions(sessions):
     from collections import Counter
     c = Counter(f"{s.desc.upscaled_width}x{s.desc.frame_height} {s.desc.bit_depth}-bit" for s in sessions)
@@ -283,6 +364,8 @@ def main():
     ap.add_argument("--side", action="store_true",
                     help="time the side paths on one 3840x2160 10-bit frame: side information built on the "
                          "GPU from walker grids, and the show_existing_frame grain output of a resident slot")
+    ap.add_argument("--elect", action="store_true",
+                    help="time the encoder's CDEF strength election on 1080p and 4K 10-bit frames")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run only the plain device copy kernel over this many bytes, --steps times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters against a known traffic)")
@@ -325,6 +408,9 @@ def main():
         return
     if args.side:
         print(json.dumps(side_paths()))
+        return
+    if args.elect:
+        print(json.dumps(elect_bench(cpu=not args.no_cpu_baseline)))
         return
 
     seeds = workload_seeds(args.cfg, args.frames)

@@ -551,6 +551,87 @@ int oxav1_pf_server_submit_existing(oxav1_pf_server *s, uint32_t session, uint32
 int oxav1_pf_server_poll(oxav1_pf_server *s, oxav1_server_done *done, uint32_t cap, uint32_t *n_done);
 int oxav1_pf_server_stats(oxav1_pf_server *s, oxav1_server_stats *st);
 
+/* ---------------------------------------------------------------------- */
+/* Encoder elections on the filter kernels                                 */
+/* ---------------------------------------------------------------------- */
+
+/* One CDEF strength candidate for one plane set (plane_set 0: luma,
+ * 1: both chroma planes), as the encoder's search evaluates it
+ * (encoder/cdef_elect.rs:180-190 `params_for`): damping and the set's
+ * (pri, sec) strengths in strength slot 0, every other slot zero. */
+typedef struct oxav1_cdef_cand {
+    uint8_t damping;
+    uint8_t plane_set;
+    uint8_t pri;
+    uint8_t sec;
+} oxav1_cdef_cand;
+
+/* One frame of a per-unit SSD job, all pointers in device memory. Planes
+ * are u8 when sample_bytes == 1, u16 when 2; pitches in samples. rows/cols
+ * are the plane buffers' shapes (the reference's PlaneBuffer rows/cols);
+ * the 8x8 walk over the mi grid must cover them. cdef_idx has one byte per
+ * 64x64 unit ((mi_rows+15)>>4 x (mi_cols+15)>>4), skip one per mi cell.
+ * ssd receives ncand x n_units uint64 (candidate-major). */
+typedef struct oxav1_cdef_ssd_frame {
+    uint32_t mi_rows, mi_cols;
+    const int8_t *cdef_idx;
+    const uint8_t *skip;
+    const void *recon[3];
+    const void *input[3];
+    uint32_t recon_pitch[3], input_pitch[3];
+    uint32_t rows[3], cols[3];
+    uint8_t bit_depth, subsampling_x, subsampling_y, num_planes;
+    uint8_t sample_bytes, pad[3];
+    uint64_t *ssd;
+} oxav1_cdef_ssd_frame;
+
+/* Replaces the per-candidate `run_units` pass of the CDEF election
+ * (encoder/cdef_elect.rs:192-240): for every candidate and every 64x64
+ * unit, the SSD against `input` of the unit's samples of the candidate's
+ * plane set after one §7.15 run over `recon` (units whose cdef_idx is 0
+ * filter with the candidate; other ids and skip blocks copy). The §7.15.2
+ * direction search runs once per 8x8 block for all candidates. One launch
+ * over all frames, asynchronous on `hip_stream` (NULL = the library
+ * stream, synchronised before return). ncand <= 128. */
+int oxav1_pf_cdef_unit_ssd(const oxav1_cdef_ssd_frame *frames, uint32_t nframes, const oxav1_cdef_cand *cands,
+                           uint32_t ncand, void *hip_stream);
+
+/* CdefElectInput (encoder/cdef_elect.rs:91-113). The walker's committed
+ * grids are cdef_idx (one byte per 64x64 unit, -1 = no idx coded) and skip
+ * (one byte per mi cell); planes are host u16 arrays of width x height
+ * (luma) and chroma_w x chroma_h. */
+typedef struct oxav1_cdef_elect_input {
+    uint32_t mi_rows, mi_cols;
+    const int8_t *cdef_idx;
+    const uint8_t *skip;
+    const uint16_t *input[3];
+    const uint16_t *recon[3];
+    uint32_t width, height, chroma_w, chroma_h;
+    uint8_t bit_depth, subsampling_x, subsampling_y, num_planes;
+    uint8_t max_bits, pad[3];
+    uint64_t lambda;
+} oxav1_cdef_elect_input;
+
+/* CdefElection (encoder/cdef_elect.rs:72-89). elected = 0 is the
+ * reference's None (no coded unit, or the unfiltered frame won). The
+ * unit-id arrays are caller-owned, n_units = ((mi_rows+15)>>4) x
+ * ((mi_cols+15)>>4) entries each, written when the plan exists. */
+typedef struct oxav1_cdef_election {
+    int32_t elected;
+    int32_t has_frame_level;
+    oxav1_cdef_params best;
+    oxav1_cdef_params frame_level;
+    uint64_t best_d, frame_level_d, base_d;
+    int8_t *best_unit_idx;
+    int8_t *frame_level_unit_idx;
+} oxav1_cdef_election;
+
+/* Replaces elect_cdef (encoder/cdef_elect.rs:136): the same candidate
+ * sweeps, damping refinement, greedy per-unit set growth and D*256 +
+ * lambda*R256 choice, with every candidate's per-unit SSD table computed
+ * on the GPU (three launches of oxav1_pf_cdef_unit_ssd). Synchronous. */
+int oxav1_pf_cdef_elect(const oxav1_cdef_elect_input *in, oxav1_cdef_election *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,8 @@ _SIGS = {
     "oo_chain": (C.c_int, [_P(abi.FrameDesc), _P(OoChainGrids), _P(abi.Plane), _P(abi.Plane),
                            _P(abi.Plane), _P(C.c_double)]),
     "oo_table_checksum": (C.c_int64, [C.c_char_p]),
+    "oo_cdef_elect": (C.c_int, [_P(abi.CdefElectInput), _P(abi.CdefElection)]),
+    "oo_cdef_run_units": (None, [_P(abi.CdefElectInput), _P(abi.CdefParams), C.c_int, _P(C.c_uint64)]),
 }
 
 _lib = None

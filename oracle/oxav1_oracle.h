@@ -70,6 +70,11 @@ void oo_cdef_filter_block(const oxav1_cdef_ctx *ctx, const oxav1_plane_i32 *src,
                           int32_t dir);
 int32_t oo_cdef_constrain(int32_t diff, int32_t threshold, int32_t damping);
 
+/* ---- encoder CDEF election (encoder/cdef_elect.rs), oo_elect.c ---- */
+int oo_cdef_elect(const oxav1_cdef_elect_input *in, oxav1_cdef_election *out);
+void oo_cdef_run_units(const oxav1_cdef_elect_input *in, const oxav1_cdef_params *params, int which,
+                       uint64_t *ssd);
+
 /* ---- §7.16 superres (superres.rs) ---- */
 int oo_upscale_frame(const oxav1_sr_ctx *ctx, const oxav1_plane_i32 *in, uint32_t nin,
                      oxav1_plane_i32 *out, uint32_t nout);

@@ -175,12 +175,13 @@ __device__ __forceinline__ CdefTaps cdef_taps(const CdefBlk &b, bool chroma, int
 // Filter one plane of a unit from an LDS tile with a kCdefB border
 // (cdef_filter_block, cdef.rs:465-590).  Lanes own columns: 1 << row_log2
 // consecutive threads cover one tile row, and each thread walks down its
-// column in steps of blockDim.x >> row_log2 rows.
+// column in steps of blockDim.x >> row_log2 rows.  Every filtered sample
+// goes to sink(yy, xx, py, px, value) (tile and plane coordinates).
 // CHECK: per-tap "inside the plane" test (units touching the frame edge).
-template <typename L, typename T, bool CHECK>
-__device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
-                                  int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
-                                  int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
+template <typename L, bool CHECK, typename Sink>
+__device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
+                                                 int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
+                                                 int32_t row_log2, Sink &&sink) {
     const int xx = threadIdx.x & ((1 << row_log2) - 1);
     const int ystep = blockDim.x >> row_log2;
     if (xx >= tw) return;
@@ -224,8 +225,16 @@ __device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, in
             const int32_t delta = (8 + sum - (sum < 0 ? 1 : 0)) >> 4;
             out = clip3(mn, mx, x + delta);
         }
-        dst[(size_t)py * dpitch + px] = (T)out;
+        sink(yy, xx, py, px, out);
     }
+}
+
+template <typename L, typename T, bool CHECK>
+__device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
+                                  int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
+                                  int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
+    cdef_filter_tile<L, CHECK>(t, ls, tw, th, ox, oy, pw, ph, bw_log2, bh_log2, taps, row_log2,
+                               [&](int, int, int32_t py, int32_t px, int32_t v) { dst[(size_t)py * dpitch + px] = (T)v; });
 }
 
 // Exact i / d for 0 <= i < 2^20 / d with m = ceil(2^20 / d), d <= 128.

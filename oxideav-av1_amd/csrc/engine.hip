@@ -1332,7 +1332,12 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_walker_grids") return sizeof(oxav1_walker_grids);
     if (n == "oxav1_server_done") return sizeof(oxav1_server_done);
     if (n == "oxav1_server_stats") return sizeof(oxav1_server_stats);
+    if (n == "oxav1_cdef_cand") return sizeof(oxav1_cdef_cand);
+    if (n == "oxav1_cdef_ssd_frame") return sizeof(oxav1_cdef_ssd_frame);
+    if (n == "oxav1_cdef_elect_input") return sizeof(oxav1_cdef_elect_input);
+    if (n == "oxav1_cdef_election") return sizeof(oxav1_cdef_election);
     return 0;
 }
 
 #include "dropin.inc"
+#include "elect.inc"

@@ -186,6 +186,37 @@ class ServerStats(C.Structure):
     ]
 
 
+class CdefCand(C.Structure):
+    _fields_ = [("damping", u8), ("plane_set", u8), ("pri", u8), ("sec", u8)]
+
+
+class CdefSsdFrame(C.Structure):
+    _fields_ = [
+        ("mi_rows", u32), ("mi_cols", u32), ("cdef_idx", C.c_void_p), ("skip", C.c_void_p),
+        ("recon", C.c_void_p * 3), ("input", C.c_void_p * 3), ("recon_pitch", u32 * 3),
+        ("input_pitch", u32 * 3), ("rows", u32 * 3), ("cols", u32 * 3), ("bit_depth", u8),
+        ("subsampling_x", u8), ("subsampling_y", u8), ("num_planes", u8), ("sample_bytes", u8),
+        ("pad", u8 * 3), ("ssd", C.c_void_p),
+    ]
+
+
+class CdefElectInput(C.Structure):
+    _fields_ = [
+        ("mi_rows", u32), ("mi_cols", u32), ("cdef_idx", C.c_void_p), ("skip", C.c_void_p),
+        ("input", C.c_void_p * 3), ("recon", C.c_void_p * 3), ("width", u32), ("height", u32),
+        ("chroma_w", u32), ("chroma_h", u32), ("bit_depth", u8), ("subsampling_x", u8),
+        ("subsampling_y", u8), ("num_planes", u8), ("max_bits", u8), ("pad", u8 * 3), ("lambda_", u64),
+    ]
+
+
+class CdefElection(C.Structure):
+    _fields_ = [
+        ("elected", C.c_int32), ("has_frame_level", C.c_int32), ("best", CdefParams),
+        ("frame_level", CdefParams), ("best_d", u64), ("frame_level_d", u64), ("base_d", u64),
+        ("best_unit_idx", C.c_void_p), ("frame_level_unit_idx", C.c_void_p),
+    ]
+
+
 ABI_STRUCTS = {
     "oxav1_plane_i32": Plane, "oxav1_lf_ctx": LfCtx, "oxav1_lf_grids": LfGrids,
     "oxav1_cdef_params": CdefParams, "oxav1_cdef_ctx": CdefCtx, "oxav1_cdef_grids": CdefGrids,
@@ -193,7 +224,8 @@ ABI_STRUCTS = {
     "oxav1_lr_grids": LrGrids, "oxav1_film_grain_params": FilmGrainParams,
     "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev,
     "oxav1_tuning": Tuning, "oxav1_walker_grids": WalkerGrids, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
-    "oxav1_server_stats": ServerStats,
+    "oxav1_server_stats": ServerStats, "oxav1_cdef_cand": CdefCand, "oxav1_cdef_ssd_frame": CdefSsdFrame,
+    "oxav1_cdef_elect_input": CdefElectInput, "oxav1_cdef_election": CdefElection,
 }
 
 _P = C.POINTER
@@ -240,6 +272,8 @@ _SIGS = {
         C.c_int, [C.c_void_p, u32, u32, _P(FilmGrainParams), C.c_double, _P(u64)]),
     "oxav1_pf_server_poll": (C.c_int, [C.c_void_p, _P(ServerDone), u32, _P(u32)]),
     "oxav1_pf_server_stats": (C.c_int, [C.c_void_p, _P(ServerStats)]),
+    "oxav1_pf_cdef_unit_ssd": (C.c_int, [_P(CdefSsdFrame), u32, _P(CdefCand), u32, C.c_void_p]),
+    "oxav1_pf_cdef_elect": (C.c_int, [_P(CdefElectInput), _P(CdefElection)]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
     "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -1,0 +1,146 @@
+"""Encoder elections on the filter kernels: the CDEF strength election.
+
+`elect_cdef` mirrors the reference's `elect_cdef` (encoder/cdef_elect.rs:136)
+over the C ABI entry point `oxav1_pf_cdef_elect`: the committed walker grids
+(`cdef_idx` per 64x64 unit, -1 where no idx is coded; `skip` per mi cell),
+the source and pre-CDEF reconstruction planes (u16), lambda and the highest
+`cdef_bits` the search may propose.  It returns None where the reference
+returns None (no coded unit, or the unfiltered frame wins), otherwise a
+`CdefElection` with the best plan, the frame-level fallback when present,
+and the unfiltered SSD.  `lib` may be any library exporting the same
+symbol with the same struct (the tests pass the oracle's).
+
+`synthetic_input` builds an election input of a given shape from a seed:
+a source with edges and texture, and a reconstruction with quantisation
+noise and ringing along the edges.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class CdefPlan:
+    params: dict
+    unit_idx: np.ndarray
+    d: int
+
+
+@dataclass
+class CdefElection:
+    best: CdefPlan
+    frame_level: CdefPlan = None
+    base_d: int = 0
+
+
+@dataclass
+class ElectInput:
+    mi_rows: int
+    mi_cols: int
+    cdef_idx: np.ndarray  # int8, one per 64x64 unit
+    skip: np.ndarray      # uint8, one per mi cell
+    input: list           # u16 planes
+    recon: list           # u16 planes
+    bit_depth: int
+    subsampling_x: int
+    subsampling_y: int
+    lambda_: int = 0
+    max_bits: int = 0
+    keep: list = field(default_factory=list)
+
+    @property
+    def num_planes(self):
+        return len(self.recon)
+
+    @property
+    def n_units(self):
+        return ((self.mi_rows + 15) // 16) * ((self.mi_cols + 15) // 16)
+
+    def struct(self):
+        s = abi.CdefElectInput()
+        s.mi_rows, s.mi_cols = self.mi_rows, self.mi_cols
+        arrs = [np.ascontiguousarray(self.cdef_idx, dtype=np.int8), np.ascontiguousarray(self.skip, dtype=np.uint8)]
+        s.cdef_idx, s.skip = arrs[0].ctypes.data, arrs[1].ctypes.data
+        for p in range(self.num_planes):
+            a = np.ascontiguousarray(self.input[p], dtype=np.uint16)
+            b = np.ascontiguousarray(self.recon[p], dtype=np.uint16)
+            arrs += [a, b]
+            s.input[p], s.recon[p] = a.ctypes.data, b.ctypes.data
+        h, w = self.recon[0].shape
+        s.width, s.height = w, h
+        if self.num_planes > 1:
+            s.chroma_h, s.chroma_w = self.recon[1].shape
+        s.bit_depth = self.bit_depth
+        s.subsampling_x, s.subsampling_y = self.subsampling_x, self.subsampling_y
+        s.num_planes = self.num_planes
+        s.max_bits = self.max_bits
+        s.lambda_ = self.lambda_
+        self.keep = arrs
+        return s
+
+
+def params_dict(p):
+    return {"cdef_damping": p.cdef_damping, "cdef_bits": p.cdef_bits,
+            "y_pri": list(p.y_pri_strength), "y_sec": list(p.y_sec_strength),
+            "uv_pri": list(p.uv_pri_strength), "uv_sec": list(p.uv_sec_strength)}
+
+
+def elect_cdef(inp, lib=None, fn="oxav1_pf_cdef_elect"):
+    """The election for one frame (see the module docstring)."""
+    lib = lib or abi.load()
+    s = inp.struct()
+    out = abi.CdefElection()
+    best_ids = np.zeros(inp.n_units, dtype=np.int8)
+    fl_ids = np.zeros(inp.n_units, dtype=np.int8)
+    out.best_unit_idx = best_ids.ctypes.data
+    out.frame_level_unit_idx = fl_ids.ctypes.data
+    st = getattr(lib, fn)(C.byref(s), C.byref(out))
+    if st:
+        raise abi.Oxav1Error(st, fn)
+    if not out.elected:
+        return None
+    fl = None
+    if out.has_frame_level:
+        fl = CdefPlan(params_dict(out.frame_level), fl_ids, out.frame_level_d)
+    return CdefElection(CdefPlan(params_dict(out.best), best_ids, out.best_d), fl, out.base_d)
+
+
+def synthetic_input(seed, width, height, bit_depth=8, ssx=1, ssy=1, mono=False, skip_frac=0.25,
+                    uncoded_frac=0.1, noise=None, lambda_=None, max_bits=3):
+    """An election input with edges, texture, quantisation noise and ringing."""
+    rng = np.random.default_rng(seed)
+    mx = (1 << bit_depth) - 1
+    sc = 1 << (bit_depth - 8)
+    noise = 3 * sc if noise is None else noise
+
+    def plane(h, w):
+        y, x = np.mgrid[0:h, 0:w]
+        base = 96 + 40 * np.sin(x / (7 + rng.uniform(0, 9))) * np.cos(y / (5 + rng.uniform(0, 9)))
+        edges = np.zeros((h, w))
+        for _ in range(6):
+            a, b, c = rng.normal(size=3)
+            edges += 50 * ((a * x + b * y + c * 40) > 0)
+        src = np.clip((base + edges + rng.normal(0, 2, (h, w))) * sc, 0, mx)
+        # ringing: a decaying oscillation after every edge transition
+        grad = np.abs(np.diff(edges, axis=1, prepend=edges[:, :1])) > 0
+        ring = np.zeros((h, w))
+        for k in range(1, 6):
+            ring[:, k:] += grad[:, :-k] * (8 * sc) * ((-1) ** k) / k
+        rec = np.clip(src + ring + rng.normal(0, noise, (h, w)), 0, mx)
+        return src.astype(np.uint16), rec.astype(np.uint16)
+
+    pl = [plane(height, width)]
+    if not mono:
+        ch, cw = (height + ssy) >> ssy, (width + ssx) >> ssx
+        pl += [plane(ch, cw), plane(ch, cw)]
+    mi_rows, mi_cols = (height + 3) // 4, (width + 3) // 4
+    n_units = ((mi_rows + 15) // 16) * ((mi_cols + 15) // 16)
+    cdef_idx = np.where(rng.random(n_units) < uncoded_frac, -1, 0).astype(np.int8)
+    skip = (rng.random(mi_rows * mi_cols) < skip_frac).astype(np.uint8)
+    if lambda_ is None:
+        lambda_ = int(rng.integers(20, 4000)) * sc * sc
+    return ElectInput(mi_rows, mi_cols, cdef_idx, skip, [p[0] for p in pl], [p[1] for p in pl], bit_depth,
+                      ssx if not mono else 1, ssy if not mono else 1, lambda_, max_bits)

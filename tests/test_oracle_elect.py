@@ -1,0 +1,205 @@
+"""Pins of the oracle's CDEF strength election (encoder/cdef_elect.rs).
+
+The election's arithmetic is the §7.15 filter (pinned elsewhere) plus sums of
+squared errors, a fixed candidate search and a rate model.  These tests check
+the oracle's election against what fixes it independently of its code:
+
+  - the per-unit SSD tables equal a numpy sum over the oracle's cdef_frame
+    output (run_units, cdef_elect.rs:192-240);
+  - every returned plan's D equals the SSD of applying that plan through
+    cdef_frame with the plan's unit grid (the reference's CdefPlan contract,
+    cdef_elect.rs:57-62), and base_d the SSD of the unfiltered frame;
+  - the winner's D*256 + lambda*R256 beats the unfiltered frame (which wins
+    ties) and is no worse than the frame-level fallback;
+  - the frame-level arm (cdef_bits 0) is no worse than a brute-force minimum
+    over the coarse candidate grid at dampings 3 and 5, and is elected
+    exactly when that minimum beats the unfiltered frame;
+  - per-unit plans assign every coded unit the first set of least cost, and
+    uncoded units -1;
+  - ringing along edges elects the filter and lowers the error (the
+    reference's tripwire, tests/cdef_ab.rs:132-140).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oxav1 import abi
+from oxav1.elect import elect_cdef, synthetic_input
+from oxav1.frames import planes_struct
+
+
+@pytest.fixture(scope="module")
+def oo(oracle_mod):
+    return oracle_mod.load()
+
+
+def unit_ssd(inp, planes, which):
+    """Per-64x64-unit SSD of plane set `which` of `planes` against the source."""
+    sb_cols = (inp.mi_cols + 15) // 16
+    out = np.zeros(inp.n_units, dtype=np.uint64)
+    idx = [0] if which == 0 else ([1, 2] if inp.num_planes > 1 else [])
+    for p in idx:
+        h, w = planes[p].shape
+        sy, sx = (inp.subsampling_y, inp.subsampling_x) if p else (0, 0)
+        y, x = np.mgrid[0:h, 0:w]
+        unit = ((y << sy) // 64) * sb_cols + (x << sx) // 64
+        d = planes[p].astype(np.int64) - inp.input[p].astype(np.int64)
+        np.add.at(out, unit.ravel(), (d * d).ravel().astype(np.uint64))
+    return out
+
+
+def oracle_cdef(oo, inp, params, unit_idx):
+    """cdef_frame over the reconstruction with `params` and a per-unit grid."""
+    ctx = abi.CdefCtx()
+    ctx.mi_rows, ctx.mi_cols = inp.mi_rows, inp.mi_cols
+    ctx.num_planes = inp.num_planes
+    ctx.bit_depth = inp.bit_depth
+    ctx.subsampling_x, ctx.subsampling_y = inp.subsampling_x, inp.subsampling_y
+    ctx.params = params
+    ids = np.ascontiguousarray(unit_idx, dtype=np.int8)
+    skip = np.ascontiguousarray(inp.skip, dtype=np.uint8)
+    g = abi.CdefGrids(ids.ctypes.data_as(C.POINTER(C.c_int8)), skip.ctypes.data_as(C.POINTER(C.c_uint8)))
+    src = [p.astype(np.int32) for p in inp.recon]
+    dst = [p.copy() for p in src]
+    oo.oo_cdef_frame(C.byref(ctx), C.byref(g), planes_struct(src), len(src), planes_struct(dst), len(dst))
+    return dst
+
+
+def params_struct(d):
+    p = abi.CdefParams()
+    p.cdef_damping, p.cdef_bits = d["cdef_damping"], d["cdef_bits"]
+    for i in range(8):
+        p.y_pri_strength[i], p.y_sec_strength[i] = d["y_pri"][i], d["y_sec"][i]
+        p.uv_pri_strength[i], p.uv_sec_strength[i] = d["uv_pri"][i], d["uv_sec"][i]
+    return p
+
+
+def simple_params(damping, y=(0, 0), uv=(0, 0)):
+    p = abi.CdefParams()
+    p.cdef_damping = damping
+    p.y_pri_strength[0], p.y_sec_strength[0] = y
+    p.uv_pri_strength[0], p.uv_sec_strength[0] = uv
+    return p
+
+
+def total_ssd(inp, planes):
+    return int(unit_ssd(inp, planes, 0).sum() + unit_ssd(inp, planes, 1).sum())
+
+
+def rate256(params, inp):
+    bits = params["cdef_bits"]
+    if bits == 0:
+        return 0
+    coded = int((np.asarray(inp.cdef_idx) != -1).sum())
+    hdr = 12 if inp.num_planes > 1 else 6
+    return 256 * (bits * coded + ((1 << bits) - 1) * hdr)
+
+
+CASES = [  # (seed, w, h, bit depth, ssx, ssy, mono, max_bits)
+    (1, 128, 96, 8, 1, 1, False, 0),
+    (2, 136, 72, 10, 1, 1, False, 3),
+    (3, 96, 104, 8, 1, 0, False, 2),
+    (4, 104, 64, 12, 0, 0, False, 1),
+    (5, 120, 80, 8, 1, 1, True, 3),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_run_units_tables_match_numpy(oo, case):
+    seed, w, h, bd, ssx, ssy, mono, mb = case
+    inp = synthetic_input(seed, w, h, bd, ssx, ssy, mono, max_bits=mb)
+    rng = np.random.default_rng(seed)
+    s = inp.struct()
+    for _ in range(3):
+        p = simple_params(int(rng.integers(3, 7)), (int(rng.integers(0, 16)), int(rng.choice([0, 1, 2, 4]))),
+                          (int(rng.integers(0, 16)), int(rng.choice([0, 1, 2, 4]))))
+        ref = oracle_cdef(oo, inp, p, inp.cdef_idx)
+        for which in (0, 1):
+            got = (C.c_uint64 * inp.n_units)()
+            oo.oo_cdef_run_units(C.byref(s), C.byref(p), which, got)
+            assert np.array_equal(np.array(got, dtype=np.uint64), unit_ssd(inp, ref, which))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_plans_are_exact_and_win(oo, case):
+    seed, w, h, bd, ssx, ssy, mono, mb = case
+    inp = synthetic_input(seed, w, h, bd, ssx, ssy, mono, max_bits=mb)
+    r = elect_cdef(inp, oo, "oo_cdef_elect")
+    base = total_ssd(inp, [p.astype(np.int32) for p in inp.recon])
+    assert r is not None, "ringing content must elect the filter"
+    assert r.base_d == base
+    for plan in [r.best] + ([r.frame_level] if r.frame_level else []):
+        out = oracle_cdef(oo, inp, params_struct(plan.params), plan.unit_idx)
+        assert plan.d == total_ssd(inp, out)
+        assert np.array_equal(plan.unit_idx == -1, np.asarray(inp.cdef_idx) == -1)
+        assert plan.params["cdef_bits"] <= mb
+    score = r.best.d * 256 + inp.lambda_ * rate256(r.best.params, inp)
+    assert score < base * 256
+    if r.frame_level:
+        assert r.frame_level.d < base and r.frame_level.params["cdef_bits"] == 0
+        assert score <= r.frame_level.d * 256
+        assert r.best.params["cdef_bits"] > 0
+
+
+COARSE = [(0, 0)] + [(p, s) for p in (1, 2, 3, 4, 6, 9, 12, 15) for s in (0, 2)] + [(0, 1), (0, 2), (0, 4)]
+
+
+@pytest.mark.parametrize("seed,mono,noise", [(11, False, None), (12, True, None), (13, False, 0), (14, False, 40)])
+def test_frame_level_vs_brute_force(oo, seed, mono, noise):
+    inp = synthetic_input(seed, 72, 64, 8, 1, 1, mono, noise=noise, max_bits=0)
+    base = total_ssd(inp, [p.astype(np.int32) for p in inp.recon])
+    best = None
+    for d in (3, 5):
+        ly = min(unit_ssd(inp, oracle_cdef(oo, inp, simple_params(d, y=s), inp.cdef_idx), 0).sum() for s in COARSE)
+        luv = 0
+        if not mono:
+            luv = min(unit_ssd(inp, oracle_cdef(oo, inp, simple_params(d, uv=s), inp.cdef_idx), 1).sum()
+                      for s in COARSE)
+        best = min(best if best is not None else ly + luv, ly + luv)
+    r = elect_cdef(inp, oo, "oo_cdef_elect")
+    if best < base:
+        assert r is not None and r.best.params["cdef_bits"] == 0 and r.best.d <= best
+    else:
+        # the coarse grid cannot beat the unfiltered frame; the refinements may
+        assert r is None or r.best.d < base
+
+
+def test_per_unit_choice_is_argmin(oo):
+    found = 0
+    for seed in (21, 22, 23):
+        inp = synthetic_input(seed, 256, 192, 8, 1, 1, lambda_=0, max_bits=3)
+        # vary the content per unit: the right half is reconstructed exactly
+        # (any filtering there adds error), the left half keeps its noise
+        for p in range(3):
+            w = inp.recon[p].shape[1]
+            inp.recon[p][:, w // 2:] = inp.input[p][:, w // 2:]
+        r = elect_cdef(inp, oo, "oo_cdef_elect")
+        if r is None or r.best.params["cdef_bits"] == 0:
+            continue
+        found += 1
+        P = r.best.params
+        nsets = 1 << P["cdef_bits"]
+        s = inp.struct()
+        cost = np.zeros((nsets, inp.n_units), dtype=np.uint64)
+        for k in range(nsets):
+            p = simple_params(P["cdef_damping"], (P["y_pri"][k], P["y_sec"][k]), (P["uv_pri"][k], P["uv_sec"][k]))
+            for which in (0, 1):
+                t = (C.c_uint64 * inp.n_units)()
+                oo.oo_cdef_run_units(C.byref(s), C.byref(p), which, t)
+                cost[k] += np.array(t, dtype=np.uint64)
+        coded = np.asarray(inp.cdef_idx) != -1
+        assert np.array_equal(r.best.unit_idx[coded], np.argmin(cost[:, coded], axis=0))
+        assert (r.best.unit_idx[~coded] == -1).all()
+        base = unit_ssd(inp, inp.recon, 0) + unit_ssd(inp, inp.recon, 1)
+        assert r.best.d == int(cost.min(axis=0)[coded].sum() + base[~coded].sum())
+    assert found, "no case elected a per-unit plan"
+
+
+def test_ringing_tripwire_and_no_coded_units(oo):
+    inp = synthetic_input(31, 128, 128, 10, 1, 1, noise=0, max_bits=0)
+    r = elect_cdef(inp, oo, "oo_cdef_elect")
+    assert r is not None and r.best.d < r.base_d
+    assert max(r.best.params["y_pri"][0], r.best.params["y_sec"][0]) > 0
+    inp.cdef_idx[:] = -1
+    assert elect_cdef(inp, oo, "oo_cdef_elect") is None
