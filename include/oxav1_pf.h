@@ -632,6 +632,33 @@ typedef struct oxav1_cdef_election {
  * on the GPU (three launches of oxav1_pf_cdef_unit_ssd). Synchronous. */
 int oxav1_pf_cdef_elect(const oxav1_cdef_elect_input *in, oxav1_cdef_election *out);
 
+/* Superres election helpers (encoder/superres_elect.rs). Planes are host
+ * u16 arrays, row-major without padding. */
+
+/* superres_coded_width (superres_elect.rs:23-29): the §5.9.8 FrameWidth. */
+uint32_t oxav1_pf_superres_coded_width(uint32_t upscaled_width, uint32_t denom);
+/* candidate_denoms (superres_elect.rs:31-45): writes up to 8 denominators
+ * to `denoms` and returns their number. */
+uint32_t oxav1_pf_superres_candidate_denoms(uint32_t width, uint32_t *denoms);
+/* superres_probe (superres_elect.rs:62-91): *probe = 1 when the mean
+ * absolute horizontal luma second difference is under 10/16 of an 8-bit
+ * step. Computed on the GPU. */
+int oxav1_pf_superres_probe(const uint16_t *y, uint32_t width, uint32_t height, uint8_t bit_depth,
+                            int32_t *probe);
+/* downscale_width (superres_elect.rs:93-150): every plane low-passed
+ * [1,2,1]/4 and resampled centre-aligned at 1/64-sample precision, luma
+ * from width to coded_width columns, chroma from width >> ssx to
+ * coded_width >> ssx (heights height and height >> ssy). coded_width must
+ * be below width and above 0. Computed on the GPU. */
+int oxav1_pf_superres_downscale(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx,
+                                uint8_t ssy, uint8_t num_planes, uint32_t coded_width, uint16_t *const dst[3]);
+/* upscale_recon (superres_elect.rs:152-232): §7.16 upscaling of u16
+ * planes ((width + ssx) >> ssx etc.) through the superres kernel, errors
+ * as oxav1_pf_upscale_frame. */
+int oxav1_pf_superres_upscale_recon(const uint16_t *const planes[3], uint32_t width, uint32_t height,
+                                    uint32_t upscaled_width, uint32_t mi_cols, uint8_t bit_depth, uint8_t ssx,
+                                    uint8_t ssy, uint8_t num_planes, uint16_t *const out[3]);
+
 #ifdef __cplusplus
 }
 #endif

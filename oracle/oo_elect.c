@@ -455,3 +455,111 @@ void oo_cdef_run_units(const oxav1_cdef_elect_input *in, const oxav1_cdef_params
         free(R.dst[p]);
     }
 }
+
+/* ===================================================================== */
+/* Superres election helpers (encoder/superres_elect.rs)                  */
+/* ===================================================================== */
+
+/* superres_coded_width (superres_elect.rs:23-29): SUPERRES_NUM = 8 */
+uint32_t oo_superres_coded_width(uint32_t upscaled_width, uint32_t denom) {
+    return (upscaled_width * 8 + denom / 2) / denom;
+}
+
+/* candidate_denoms (superres_elect.rs:31-45): SUPERRES_DENOM_MIN = 9 */
+uint32_t oo_superres_candidate_denoms(uint32_t width, uint32_t *out) {
+    uint32_t n = 0;
+    for (uint32_t d = 9; d <= 16; d++) {
+        uint32_t wd = oo_superres_coded_width(width, d);
+        if (wd >= 8 && wd % 8 == 0 && wd < width) out[n++] = d;
+    }
+    return n;
+}
+
+/* superres_probe (superres_elect.rs:62-91) */
+int oo_superres_probe(const uint16_t *y, uint32_t w, uint32_t h, uint8_t bit_depth) {
+    if (w < 3) return 0;
+    uint64_t sum = 0, count = 0;
+    for (size_t r = 0; r < h; r++)
+        for (size_t c = 1; c < (size_t)w - 1; c++) {
+            int64_t m = y[r * w + c];
+            int64_t v = 2 * m - (int64_t)y[r * w + c - 1] - (int64_t)y[r * w + c + 1];
+            sum += (uint64_t)(v < 0 ? -v : v);
+            count++;
+        }
+    if (count == 0) return 0;
+    uint64_t mean16 = (sum * 16) / count / (1ull << (bit_depth - 8));
+    return mean16 < 10;
+}
+
+/* downscale_plane_width (superres_elect.rs:125-150) */
+static void downscale_plane_width(const uint16_t *src, size_t w_src, size_t h, size_t w_dst, uint16_t *out) {
+    uint32_t *lp = (uint32_t *)malloc(sizeof(uint32_t) * (w_src ? w_src : 1));
+    for (size_t r = 0; r < h; r++) {
+        const uint16_t *row = src + r * w_src;
+        for (size_t x = 0; x < w_src; x++) {
+            uint32_t l = row[x ? x - 1 : 0], m = row[x], rr = row[x + 1 < w_src ? x + 1 : w_src - 1];
+            lp[x] = (l + 2 * m + rr + 2) >> 2;
+        }
+        for (size_t x = 0; x < w_dst; x++) {
+            uint64_t s64 = ((2 * (uint64_t)x + 1) * (uint64_t)w_src * 32) / (uint64_t)w_dst - 32;
+            size_t i0 = (size_t)(s64 >> 6);
+            uint32_t frac = (uint32_t)(s64 & 63);
+            size_t i1 = i0 + 1 < w_src ? i0 + 1 : w_src - 1;
+            out[r * w_dst + x] = (uint16_t)(((64 - frac) * lp[i0] + frac * lp[i1] + 32) >> 6);
+        }
+    }
+    free(lp);
+}
+
+/* downscale_width (superres_elect.rs:93-123): luma to coded_width, chroma
+ * from width >> ssx to coded_width >> ssx columns, heights unchanged. */
+void oo_superres_downscale(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx, uint8_t ssy,
+                           uint8_t num_planes, uint32_t coded_width, uint16_t *const dst[3]) {
+    downscale_plane_width(src[0], width, height, coded_width, dst[0]);
+    if (num_planes > 1)
+        for (int p = 1; p < 3; p++)
+            downscale_plane_width(src[p], width >> ssx, height >> ssy, coded_width >> ssx, dst[p]);
+}
+
+/* upscale_recon (superres_elect.rs:152-232): §7.16 on u16 planes of
+ * (width + ssx) >> ssx etc., narrowed back to u16. */
+int oo_superres_upscale_recon(const uint16_t *const planes[3], uint32_t width, uint32_t height,
+                              uint32_t upscaled_width, uint32_t mi_cols, uint8_t bit_depth, uint8_t ssx, uint8_t ssy,
+                              uint8_t num_planes, uint16_t *const out[3]) {
+    oxav1_sr_ctx ctx;
+    memset(&ctx, 0, sizeof ctx);
+    ctx.use_superres = 1;
+    ctx.frame_width = width;
+    ctx.upscaled_width = upscaled_width;
+    ctx.frame_height = height;
+    ctx.mi_cols = mi_cols;
+    ctx.num_planes = num_planes;
+    ctx.bit_depth = bit_depth;
+    ctx.subsampling_x = ssx;
+    ctx.subsampling_y = ssy;
+    oxav1_plane_i32 in[3], o[3];
+    memset(in, 0, sizeof in);
+    memset(o, 0, sizeof o);
+    for (uint32_t p = 0; p < num_planes && p < 3; p++) {
+        uint32_t pw = p ? (width + ssx) >> ssx : width, uw = p ? (upscaled_width + ssx) >> ssx : upscaled_width;
+        uint32_t ph = p ? (height + ssy) >> ssy : height;
+        in[p].rows = ph;
+        in[p].cols = pw;
+        in[p].samples = (int32_t *)malloc(sizeof(int32_t) * ((size_t)pw * ph + 1));
+        for (size_t i = 0; i < (size_t)pw * ph; i++) in[p].samples[i] = planes[p][i];
+        o[p].rows = ph;
+        o[p].cols = uw;
+        o[p].samples = (int32_t *)calloc((size_t)uw * ph + 1, sizeof(int32_t));
+    }
+    int st = oo_upscale_frame(&ctx, in, num_planes, o, num_planes);
+    for (uint32_t p = 0; p < num_planes && p < 3; p++) {
+        if (!st)
+            for (size_t i = 0; i < (size_t)o[p].rows * o[p].cols; i++) {
+                int32_t v = o[p].samples[i];
+                out[p][i] = (uint16_t)(v < 0 ? 0 : (v > 65535 ? 65535 : v));
+            }
+        free(in[p].samples);
+        free(o[p].samples);
+    }
+    return st;
+}

@@ -107,6 +107,13 @@ _SIGS = {
                            _P(abi.Plane), _P(C.c_double)]),
     "oo_table_checksum": (C.c_int64, [C.c_char_p]),
     "oo_cdef_elect": (C.c_int, [_P(abi.CdefElectInput), _P(abi.CdefElection)]),
+    "oo_superres_coded_width": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "oo_superres_candidate_denoms": (C.c_uint32, [C.c_uint32, _P(C.c_uint32)]),
+    "oo_superres_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint8]),
+    "oo_superres_downscale": (None, [C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint8, C.c_uint8,
+                                     C.c_uint32, C.c_void_p * 3]),
+    "oo_superres_upscale_recon": (C.c_int, [C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8, C.c_void_p * 3]),
     "oo_cdef_run_units": (None, [_P(abi.CdefElectInput), _P(abi.CdefParams), C.c_int, _P(C.c_uint64)]),
 }
 

@@ -75,6 +75,16 @@ int oo_cdef_elect(const oxav1_cdef_elect_input *in, oxav1_cdef_election *out);
 void oo_cdef_run_units(const oxav1_cdef_elect_input *in, const oxav1_cdef_params *params, int which,
                        uint64_t *ssd);
 
+/* ---- encoder superres election helpers (encoder/superres_elect.rs), oo_elect.c ---- */
+uint32_t oo_superres_coded_width(uint32_t upscaled_width, uint32_t denom);
+uint32_t oo_superres_candidate_denoms(uint32_t width, uint32_t *out);
+int oo_superres_probe(const uint16_t *y, uint32_t w, uint32_t h, uint8_t bit_depth);
+void oo_superres_downscale(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx, uint8_t ssy,
+                           uint8_t num_planes, uint32_t coded_width, uint16_t *const dst[3]);
+int oo_superres_upscale_recon(const uint16_t *const planes[3], uint32_t width, uint32_t height,
+                              uint32_t upscaled_width, uint32_t mi_cols, uint8_t bit_depth, uint8_t ssx, uint8_t ssy,
+                              uint8_t num_planes, uint16_t *const out[3]);
+
 /* ---- §7.16 superres (superres.rs) ---- */
 int oo_upscale_frame(const oxav1_sr_ctx *ctx, const oxav1_plane_i32 *in, uint32_t nin,
                      oxav1_plane_i32 *out, uint32_t nout);

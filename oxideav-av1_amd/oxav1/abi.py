@@ -274,6 +274,12 @@ _SIGS = {
     "oxav1_pf_server_stats": (C.c_int, [C.c_void_p, _P(ServerStats)]),
     "oxav1_pf_cdef_unit_ssd": (C.c_int, [_P(CdefSsdFrame), u32, _P(CdefCand), u32, C.c_void_p]),
     "oxav1_pf_cdef_elect": (C.c_int, [_P(CdefElectInput), _P(CdefElection)]),
+    "oxav1_pf_superres_coded_width": (u32, [u32, u32]),
+    "oxav1_pf_superres_candidate_denoms": (u32, [u32, _P(u32)]),
+    "oxav1_pf_superres_probe": (C.c_int, [C.c_void_p, u32, u32, u8, _P(C.c_int32)]),
+    "oxav1_pf_superres_downscale": (C.c_int, [C.c_void_p * 3, u32, u32, u8, u8, u8, u32, C.c_void_p * 3]),
+    "oxav1_pf_superres_upscale_recon": (
+        C.c_int, [C.c_void_p * 3, u32, u32, u32, u32, u8, u8, u8, u8, C.c_void_p * 3]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
     "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

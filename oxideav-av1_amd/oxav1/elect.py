@@ -144,3 +144,66 @@ def synthetic_input(seed, width, height, bit_depth=8, ssx=1, ssy=1, mono=False, 
         lambda_ = int(rng.integers(20, 4000)) * sc * sc
     return ElectInput(mi_rows, mi_cols, cdef_idx, skip, [p[0] for p in pl], [p[1] for p in pl], bit_depth,
                       ssx if not mono else 1, ssy if not mono else 1, lambda_, max_bits)
+
+
+# --- superres election helpers (encoder/superres_elect.rs) -------------------
+# Each takes `oracle=None` (the GPU library) or the oracle's ctypes library,
+# whose functions of the same names carry an `oo_` prefix.
+
+def _fn(oracle, name):
+    return getattr(oracle, "oo_" + name) if oracle is not None else getattr(abi.load(), "oxav1_pf_" + name)
+
+
+def _ptrs(arrays):
+    return (C.c_void_p * 3)(*([a.ctypes.data for a in arrays] + [None] * (3 - len(arrays))))
+
+
+def superres_coded_width(upscaled_width, denom, oracle=None):
+    """superres_coded_width (superres_elect.rs:23-29)."""
+    return int(_fn(oracle, "superres_coded_width")(upscaled_width, denom))
+
+
+def candidate_denoms(width, oracle=None):
+    """candidate_denoms (superres_elect.rs:31-45)."""
+    out = (C.c_uint32 * 8)()
+    n = _fn(oracle, "superres_candidate_denoms")(width, out)
+    return list(out[:n])
+
+
+def superres_probe(y, bit_depth, oracle=None):
+    """superres_probe (superres_elect.rs:62-91) on a u16 luma plane."""
+    y = np.ascontiguousarray(y, dtype=np.uint16)
+    h, w = y.shape
+    if oracle is not None:
+        return bool(oracle.oo_superres_probe(y.ctypes.data, w, h, bit_depth))
+    r = C.c_int32()
+    abi.check(abi.load().oxav1_pf_superres_probe(y.ctypes.data, w, h, bit_depth, C.byref(r)), "superres_probe")
+    return bool(r.value)
+
+
+def downscale_width(planes, ssx, ssy, coded_width, oracle=None):
+    """downscale_width (superres_elect.rs:93-150): u16 planes -> u16 planes."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    h, w = planes[0].shape
+    out = [np.zeros((h, coded_width), np.uint16)]
+    if len(planes) > 1:
+        out += [np.zeros((h >> ssy, coded_width >> ssx), np.uint16) for _ in range(2)]
+    st = _fn(oracle, "superres_downscale")(_ptrs(planes), w, h, ssx, ssy, len(planes), coded_width, _ptrs(out))
+    if oracle is None:
+        abi.check(st, "superres_downscale")
+    return out
+
+
+def upscale_recon(planes, upscaled_width, mi_cols, bit_depth, ssx, ssy, oracle=None):
+    """upscale_recon (superres_elect.rs:152-232): u16 planes at the coded
+    width -> u16 planes at upscaled_width."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    h, w = planes[0].shape
+    out = [np.zeros((h, upscaled_width), np.uint16)]
+    if len(planes) > 1:
+        out += [np.zeros(((h + ssy) >> ssy, (upscaled_width + ssx) >> ssx), np.uint16) for _ in range(2)]
+    st = _fn(oracle, "superres_upscale_recon")(_ptrs(planes), w, h, upscaled_width, mi_cols, bit_depth, ssx, ssy,
+                                               len(planes), _ptrs(out))
+    if st:
+        raise abi.Oxav1Error(st, "superres_upscale_recon") if oracle is None else RuntimeError(st)
+    return out

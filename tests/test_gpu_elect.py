@@ -3,7 +3,10 @@ the per-unit SSD tables of oxav1_pf_cdef_unit_ssd against the oracle's
 run_units for every candidate, several frames of different formats in one
 launch, u8 and u16 samples; and oxav1_pf_cdef_elect against the oracle's
 elect_cdef, field for field, across bit depths, subsamplings, monochrome,
-cdef_bits limits and lambdas."""
+cdef_bits limits and lambdas.  The superres election helpers
+(encoder/superres_elect.rs: probe, downscale, upscale of the coded-width
+reconstruction) against the oracle's on smooth and noisy planes for every
+candidate denominator."""
 import ctypes as C
 
 import numpy as np
@@ -142,3 +145,49 @@ def test_elect_none_cases(oracle_mod):
         inp.recon[p][:] = inp.input[p]  # nothing to gain: the unfiltered frame wins
     assert elect_cdef(inp) is None
     assert elect_cdef(inp, oracle_mod.load(), "oo_cdef_elect") is None
+
+
+# --- superres election helpers (encoder/superres_elect.rs) ------------------
+
+from oxav1 import elect as E  # noqa: E402
+
+SR_FORMATS = [  # (w, h, bit depth, ssx, ssy, mono)
+    (128, 64, 8, 1, 1, False), (200, 72, 10, 1, 0, False), (96, 40, 12, 0, 0, False), (136, 48, 8, 1, 1, True),
+    (1920, 1080, 10, 1, 1, False),
+]
+
+
+def _planes(rng, w, h, bd, sx, sy, mono, smooth):
+    def one(hh, ww):
+        if smooth:
+            y, x = np.mgrid[0:hh, 0:ww]
+            v = (1 << (bd - 1)) + (1 << (bd - 3)) * np.sin(x / 17.0) * np.cos(y / 11.0)
+            return np.clip(v + rng.normal(0, 1, (hh, ww)), 0, (1 << bd) - 1).astype(np.uint16)
+        return rng.integers(0, 1 << bd, (hh, ww), dtype=np.uint16)
+    pl = [one(h, w)]
+    if not mono:
+        pl += [one(h >> sy, w >> sx), one(h >> sy, w >> sx)]
+    return pl
+
+
+@pytest.mark.parametrize("fmt", SR_FORMATS)
+def test_superres_helpers_match_oracle(oracle_mod, fmt):
+    oo = oracle_mod.load()
+    w, h, bd, sx, sy, mono = fmt
+    rng = np.random.default_rng(w + h)
+    print(f"superres helpers {fmt}", flush=True)
+    for smooth in (True, False):
+        pl = _planes(rng, w, h, bd, sx, sy, mono, smooth)
+        assert E.superres_probe(pl[0], bd) == E.superres_probe(pl[0], bd, oo)
+        for d in E.candidate_denoms(w) or [16]:
+            cw = E.superres_coded_width(w, d)
+            if not 0 < cw < w:
+                continue
+            got = E.downscale_width(pl, sx, sy, cw)
+            want = E.downscale_width(pl, sx, sy, cw, oo)
+            assert all(np.array_equal(a, b) for a, b in zip(got, want)), d
+            # the coded-width reconstruction back to the full width
+            rec = [got[0]] + ([np.resize(g, (((h + sy) >> sy), ((cw + sx) >> sx))) for g in got[1:]])
+            up = E.upscale_recon(rec, w, (cw + 3) // 4, bd, sx, sy)
+            up_o = E.upscale_recon(rec, w, (cw + 3) // 4, bd, sx, sy, oo)
+            assert all(np.array_equal(a, b) for a, b in zip(up, up_o)), d

@@ -203,3 +203,85 @@ def test_ringing_tripwire_and_no_coded_units(oo):
     assert max(r.best.params["y_pri"][0], r.best.params["y_sec"][0]) > 0
     inp.cdef_idx[:] = -1
     assert elect_cdef(inp, oo, "oo_cdef_elect") is None
+
+
+# --- superres election helpers (encoder/superres_elect.rs) ------------------
+
+from oxav1 import elect as E  # noqa: E402
+
+
+@pytest.mark.parametrize("which", ["oracle", "product"])
+def test_superres_coded_width_and_candidates(oo, which):
+    """The reference's unit tests (superres_elect.rs:240-262); the product's
+    two host functions need no device."""
+    o = oo if which == "oracle" else None
+    assert E.superres_coded_width(128, 16, o) == 64
+    assert E.superres_coded_width(96, 12, o) == 64
+    assert E.superres_coded_width(96, 16, o) == 48
+    for w in range(8, 513, 8):
+        for d in E.candidate_denoms(w, o):
+            wd = E.superres_coded_width(w, d, o)
+            assert wd >= 8 and wd % 8 == 0 and wd < w, (w, d, wd)
+    assert E.candidate_denoms(128, o) == [16]
+    assert 12 in E.candidate_denoms(96, o)
+    # the spec derivation (5.9.8) for every width and denominator
+    for w in range(8, 600, 7):
+        for d in range(9, 17):
+            assert E.superres_coded_width(w, d, o) == (w * 8 + d // 2) // d
+
+
+def test_downscaler_flat_range_and_ramp(oo):
+    # superres_elect.rs:264-276: flat fields stay exact, samples stay in range
+    f = [np.full((16, 128), 173, np.uint16)] + [np.full((8, 64), 173, np.uint16)] * 2
+    d = E.downscale_width(f, 1, 1, 64, oo)
+    assert d[0].shape == (16, 64) and (d[0] == 173).all() and (d[1] == 173).all()
+    g = [(np.arange(16 * 128).reshape(16, 128) * 7 % 256).astype(np.uint16)] + [np.zeros((8, 64), np.uint16)] * 2
+    assert (E.downscale_width(g, 1, 1, 64, oo)[0] <= 255).all()
+    # a linear ramp passes the [1,2,1]/4 low-pass unchanged, so away from the
+    # edges each output is the ramp at the centre-aligned position
+    # (x + 1/2) * w_src / w_dst - 1/2, taken to 1/64 sample and rounded
+    for w_src, w_dst, a in ((128, 96, 3), (100, 72, 5), (64, 40, 1)):
+        ramp = (a * np.arange(w_src) + 7).astype(np.uint16)[None, :].repeat(4, 0)
+        out = E.downscale_width([ramp], 0, 0, w_dst, oo)[0][0].astype(np.float64)
+        x = np.arange(w_dst)
+        pos = (x + 0.5) * w_src / w_dst - 0.5
+        inner = (pos >= 1) & (pos <= w_src - 2)
+        err = np.abs(out[inner] - (a * pos[inner] + 7))
+        assert err.max() <= a / 64 + 0.5 + 1e-9
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_superres_probe(oo, bd):
+    rng = np.random.default_rng(bd)
+
+    def numpy_probe(y):
+        y = y.astype(np.int64)
+        s = np.abs(2 * y[:, 1:-1] - y[:, :-2] - y[:, 2:]).sum()
+        return (s * 16) // (y.shape[0] * (y.shape[1] - 2)) // (1 << (bd - 8)) < 10
+
+    sc = 1 << (bd - 8)
+    assert E.superres_probe(np.full((8, 8), 3 * sc, np.uint16), bd, oo)  # flat: smooth
+    cols = np.tile(np.array([0, 60 * sc], np.uint16), (8, 8))  # alternating columns
+    assert not E.superres_probe(cols, bd, oo)
+    assert not E.superres_probe(np.zeros((8, 2), np.uint16), bd, oo)  # fewer than 3 columns
+    for amp in (0, 1, 2, 3, 6, 20):
+        y = np.clip(100 * sc + rng.normal(0, amp * sc, (24, 40)), 0, (1 << bd) - 1).astype(np.uint16)
+        assert E.superres_probe(y, bd, oo) == numpy_probe(y)
+
+
+def test_upscale_recon_is_upscale_frame(oo, oracle_mod):
+    """upscale_recon is §7.16 on the u16 planes (pinned by the superres
+    tests) with the plane shapes of superres_elect.rs:184-195."""
+    rng = np.random.default_rng(5)
+    for w, h, uw, bd, sx, sy in ((64, 40, 96, 8, 1, 1), (72, 36, 120, 10, 1, 0), (48, 24, 80, 12, 0, 0)):
+        pl = [rng.integers(0, 1 << bd, (h, w), dtype=np.uint16)]
+        pl += [rng.integers(0, 1 << bd, ((h + sy) >> sy, (w + sx) >> sx), dtype=np.uint16) for _ in range(2)]
+        got = E.upscale_recon(pl, uw, (w + 3) // 4, bd, sx, sy, oo)
+        ctx = abi.SrCtx(use_superres=1, num_planes=3, bit_depth=bd, subsampling_x=sx, subsampling_y=sy,
+                        frame_width=w, upscaled_width=uw, frame_height=h, mi_cols=(w + 3) // 4)
+        src = [p.astype(np.int32) for p in pl]
+        dst = [np.zeros(g.shape, np.int32) for g in got]
+        from oxav1.frames import planes_struct
+        assert oo.oo_upscale_frame(C.byref(ctx), planes_struct(src), 3, planes_struct(dst), 3) == 0
+        for g, d in zip(got, dst):
+            assert np.array_equal(g, d)
