@@ -329,9 +329,54 @@ def elect_bench(reps=5, cpu=True):
                                  "speedup": round(cpu_s * 1e3 / statistics.median(lat), 1)}
             row["byte_exact_vs_oracle"] = bool(same)
         out.append(row)
+    # superres and film-grain election helpers, host u16 planes in and out
+    from oxav1 import Frame
+    from oxav1 import elect as E
+    helpers = []
+    for w, h in ((1920, 1080), (3840, 2160)):
+        inp = synthetic_input(0xF1 + w, w, h, 10, 1, 1)
+        pl = inp.input
+        cw = E.superres_coded_width(w, 12)
+        small = E.downscale_width(pl, 1, 1, cw)
+        rec = [small[0]] + [np.resize(p, ((h + 1) >> 1, (cw + 1) >> 1)) for p in small[1:]]
+        den = E.fg_denoise(pl, 1, 1)
+        fg = Frame.generate(2, 0x0A1F0002).desc.fg  # the 4K workload's grain parameters
+        ops = {
+            "superres_probe": lambda o=None: E.superres_probe(pl[0], 10, o),
+            "superres_downscale": lambda o=None: E.downscale_width(pl, 1, 1, cw, o),
+            "superres_upscale_recon": lambda o=None: E.upscale_recon(rec, w, (cw + 3) // 4, 10, 1, 1, o),
+            "fg_denoise": lambda o=None: E.fg_denoise(pl, 1, 1, o),
+            "fg_noise_estimate": lambda o=None: E.fg_noise_estimate(pl, den, 1, 1, 10, o),
+            "fg_apply_grain": lambda o=None: E.fg_apply_grain(fg, pl, 10, 1, 1, 2, o),
+        }
+        row = {"frame": f"{w}x{h} 10-bit 4:2:0", "gpu_ms": {}}
+        if cpu and w == 1920:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            row["cpu_oracle_ms"], row["equal"] = {}, {}
+        for name, fn in ops.items():
+            r = fn()
+            t = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                t.append((time.perf_counter() - t0) * 1e3)
+            row["gpu_ms"][name] = round(statistics.median(t), 3)
+            if "cpu_oracle_ms" in row:
+                t0 = time.perf_counter()
+                want = fn(oracle.load())
+                row["cpu_oracle_ms"][name] = round((time.perf_counter() - t0) * 1e3, 2)
+                if isinstance(r, list):
+                    row["equal"][name] = all(np.array_equal(a, b) for a, b in zip(r, want))
+                else:
+                    row["equal"][name] = r == want
+        helpers.append(row)
     return {"what": "encoder CDEF strength election (sweeps, damping refinement, greedy per-unit sets, "
                     "D*256 + lambda*R256), host u16 planes in, elected plan out",
-            "frames": out}
+            "frames": out,
+            "helpers_what": "superres and film-grain election helpers (superres_elect.rs, film_grain_elect.rs), "
+                            "host u16 planes in and out (PCIe-inclusive), superres denominator 12",
+            "helpers": helpers}
 
 
 def describe_sessions(sessions):

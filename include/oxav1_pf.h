@@ -659,6 +659,41 @@ int oxav1_pf_superres_upscale_recon(const uint16_t *const planes[3], uint32_t wi
                                     uint32_t upscaled_width, uint32_t mi_cols, uint8_t bit_depth, uint8_t ssx,
                                     uint8_t ssy, uint8_t num_planes, uint16_t *const out[3]);
 
+/* Film-grain election helpers (encoder/film_grain_elect.rs). Planes are
+ * host u16 arrays, row-major without padding; chroma planes of the
+ * denoiser and the noise estimate are (width >> ssx) x (height >> ssy),
+ * those of the grain application ((width + ssx) >> ssx) x
+ * ((height + ssy) >> ssy), as in the reference. */
+
+/* NoiseEstimate (film_grain_elect.rs:29-45): sigma x 16 on the 8-bit scale. */
+typedef struct oxav1_noise_estimate {
+    uint64_t sigma16;
+    uint64_t bin_sigma16[8];
+    uint64_t cb_sigma16;
+    uint64_t cb_bin_sigma16[4];
+    uint64_t cr_sigma16;
+    uint64_t cr_bin_sigma16[4];
+} oxav1_noise_estimate;
+
+/* denoise_frame (film_grain_elect.rs:52-94): 3x3 binomial with edge
+ * replication on every plane. Computed on the GPU. */
+int oxav1_pf_fg_denoise(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx, uint8_t ssy,
+                        uint8_t num_planes, uint16_t *const dst[3]);
+/* noise_estimate (film_grain_elect.rs:97-167): the squared-difference
+ * sums per intensity bin are reduced on the GPU, the square roots taken
+ * on the host in double precision as the reference takes them. */
+int oxav1_pf_fg_noise_estimate(const uint16_t *const input[3], const uint16_t *const denoised[3], uint32_t width,
+                               uint32_t height, uint8_t ssx, uint8_t ssy, uint8_t num_planes, uint8_t bit_depth,
+                               oxav1_noise_estimate *out);
+/* plane_sigma16 (film_grain_elect.rs:896-910) of two n-sample planes. */
+int oxav1_pf_fg_plane_sigma16(const uint16_t *a, const uint16_t *b, uint64_t n, uint8_t bit_depth,
+                              uint64_t *sigma16);
+/* apply_grain_to_recon (film_grain_elect.rs:834-893): §7.18.3 synthesis
+ * of the u16 reconstruction into `out` through the film-grain kernels. */
+int oxav1_pf_fg_apply_grain(const oxav1_film_grain_params *fg, const uint16_t *const planes[3], uint32_t width,
+                            uint32_t height, uint8_t bit_depth, uint8_t ssx, uint8_t ssy, uint8_t num_planes,
+                            uint8_t matrix_coefficients, uint16_t *const out[3]);
+
 #ifdef __cplusplus
 }
 #endif

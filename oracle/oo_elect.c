@@ -563,3 +563,129 @@ int oo_superres_upscale_recon(const uint16_t *const planes[3], uint32_t width, u
     }
     return st;
 }
+
+/* ===================================================================== */
+/* Film-grain election helpers (encoder/film_grain_elect.rs)              */
+/* ===================================================================== */
+#include <math.h>
+
+/* denoise_plane (film_grain_elect.rs:71-94): [1,2,1] x [1,2,1] / 16 with
+ * edge replication */
+static void denoise_plane(const uint16_t *src, uint16_t *dst, size_t w, size_t h) {
+    if (w == 0 || h == 0) return;
+    for (size_t r = 0; r < h; r++) {
+        size_t rm = r ? r - 1 : 0, rp = r + 1 < h ? r + 1 : h - 1;
+        for (size_t c = 0; c < w; c++) {
+            size_t cm = c ? c - 1 : 0, cp = c + 1 < w ? c + 1 : w - 1;
+#define AT(rr, cc) ((uint32_t)src[(rr) * w + (cc)])
+            uint32_t sum = AT(rm, cm) + 2 * AT(rm, c) + AT(rm, cp) + 2 * AT(r, cm) + 4 * AT(r, c) + 2 * AT(r, cp) +
+                           AT(rp, cm) + 2 * AT(rp, c) + AT(rp, cp);
+#undef AT
+            dst[r * w + c] = (uint16_t)((sum + 8) >> 4);
+        }
+    }
+}
+
+/* denoise_frame (film_grain_elect.rs:52-69): chroma planes are
+ * (width >> ssx) x (height >> ssy) */
+void oo_fg_denoise(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx, uint8_t ssy,
+                   uint8_t num_planes, uint16_t *const dst[3]) {
+    denoise_plane(src[0], dst[0], width, height);
+    if (num_planes > 1) {
+        denoise_plane(src[1], dst[1], width >> ssx, height >> ssy);
+        denoise_plane(src[2], dst[2], width >> ssx, height >> ssy);
+    }
+}
+
+/* sigma16 closure (film_grain_elect.rs:103-108) */
+static uint64_t sigma16(uint64_t sq, uint64_t count) {
+    if (count == 0) return 0;
+    return (uint64_t)llround(sqrt((double)sq / (double)count) * 16.0);
+}
+
+/* chroma_profile closure (film_grain_elect.rs:130-158) */
+static void chroma_profile(const uint16_t *src, const uint16_t *den, size_t n, uint32_t shift, uint64_t *g,
+                           uint64_t bins[4]) {
+    if (n == 0) {
+        *g = 0;
+        memset(bins, 0, 4 * sizeof(uint64_t));
+        return;
+    }
+    uint64_t sq_sum = 0, bsq[4] = {0}, bn[4] = {0};
+    for (size_t i = 0; i < n; i++) {
+        int64_t d = ((int64_t)src[i] - (int64_t)den[i]) >> shift;
+        uint64_t sq = (uint64_t)(d * d);
+        sq_sum += sq;
+        size_t bin = ((size_t)src[i] >> shift) >> 6;
+        if (bin > 3) bin = 3;
+        bsq[bin] += sq;
+        bn[bin]++;
+    }
+    *g = sigma16(sq_sum, n);
+    uint64_t minc = n / 100 > 32 ? n / 100 : 32;
+    for (int i = 0; i < 4; i++) bins[i] = bn[i] >= minc ? sigma16(bsq[i], bn[i]) : *g;
+}
+
+/* noise_estimate (film_grain_elect.rs:97-167) */
+void oo_fg_noise_estimate(const uint16_t *const input[3], const uint16_t *const denoised[3], uint32_t width,
+                          uint32_t height, uint8_t ssx, uint8_t ssy, uint8_t num_planes, uint8_t bit_depth,
+                          oxav1_noise_estimate *out) {
+    uint32_t shift = bit_depth - 8u;
+    size_t ny = (size_t)width * height;
+    uint64_t n = ny ? ny : 1;
+    uint64_t sq_sum = 0, bin_sq[8] = {0}, bin_n[8] = {0};
+    for (size_t i = 0; i < ny; i++) {
+        int64_t d = ((int64_t)input[0][i] - (int64_t)denoised[0][i]) >> shift;
+        uint64_t sq = (uint64_t)(d * d);
+        sq_sum += sq;
+        size_t bin = ((size_t)input[0][i] >> shift) >> 5;
+        if (bin > 7) bin = 7;
+        bin_sq[bin] += sq;
+        bin_n[bin]++;
+    }
+    memset(out, 0, sizeof *out);
+    out->sigma16 = sigma16(sq_sum, n);
+    uint64_t min_count = n / 100 > 64 ? n / 100 : 64;
+    for (int i = 0; i < 8; i++) out->bin_sigma16[i] = bin_n[i] >= min_count ? sigma16(bin_sq[i], bin_n[i]) : out->sigma16;
+    size_t nc = num_planes > 1 ? (size_t)(width >> ssx) * (height >> ssy) : 0;
+    chroma_profile(num_planes > 1 ? input[1] : NULL, num_planes > 1 ? denoised[1] : NULL, nc, shift, &out->cb_sigma16,
+                   out->cb_bin_sigma16);
+    chroma_profile(num_planes > 1 ? input[2] : NULL, num_planes > 1 ? denoised[2] : NULL, nc, shift, &out->cr_sigma16,
+                   out->cr_bin_sigma16);
+}
+
+/* plane_sigma16 (film_grain_elect.rs:896-910) */
+uint64_t oo_fg_plane_sigma16(const uint16_t *a, const uint16_t *b, size_t n, uint8_t bit_depth) {
+    if (n == 0) return 0;
+    uint32_t shift = bit_depth - 8u;
+    uint64_t sq = 0;
+    for (size_t i = 0; i < n; i++) {
+        int64_t d = ((int64_t)a[i] - (int64_t)b[i]) >> shift;
+        sq += (uint64_t)(d * d);
+    }
+    return (uint64_t)llround(sqrt((double)sq / (double)n) * 16.0);
+}
+
+/* apply_grain_to_recon (film_grain_elect.rs:834-893): §7.18.3 synthesis
+ * of u16 planes ((width + ssx) >> ssx etc.), narrowed with max(0) */
+void oo_fg_apply_grain(const oxav1_film_grain_params *fg, const uint16_t *const planes[3], uint32_t width,
+                       uint32_t height, uint8_t bit_depth, uint8_t ssx, uint8_t ssy, uint8_t num_planes,
+                       uint8_t matrix_coefficients, uint16_t *const out[3]) {
+    oxav1_plane_i32 b[3];
+    memset(b, 0, sizeof b);
+    for (uint32_t p = 0; p < num_planes && p < 3; p++) {
+        uint32_t pw = p ? (width + ssx) >> ssx : width, ph = p ? (height + ssy) >> ssy : height;
+        b[p].rows = ph;
+        b[p].cols = pw;
+        b[p].samples = (int32_t *)malloc(sizeof(int32_t) * ((size_t)pw * ph + 1));
+        for (size_t i = 0; i < (size_t)pw * ph; i++) b[p].samples[i] = planes[p][i];
+    }
+    oo_film_grain_synthesis(fg, bit_depth, num_planes, ssx, ssy, matrix_coefficients, b, num_planes);
+    for (uint32_t p = 0; p < num_planes && p < 3; p++) {
+        for (size_t i = 0; i < (size_t)b[p].rows * b[p].cols; i++) {
+            int32_t v = b[p].samples[i];
+            out[p][i] = (uint16_t)(v < 0 ? 0 : v);
+        }
+        free(b[p].samples);
+    }
+}

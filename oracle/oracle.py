@@ -114,6 +114,13 @@ _SIGS = {
                                      C.c_uint32, C.c_void_p * 3]),
     "oo_superres_upscale_recon": (C.c_int, [C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                             C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8, C.c_void_p * 3]),
+    "oo_fg_denoise": (None, [C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint8, C.c_uint8,
+                             C.c_void_p * 3]),
+    "oo_fg_noise_estimate": (None, [C.c_void_p * 3, C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint8,
+                                    C.c_uint8, C.c_uint8, _P(abi.NoiseEstimate)]),
+    "oo_fg_plane_sigma16": (C.c_uint64, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint8]),
+    "oo_fg_apply_grain": (None, [_P(abi.FilmGrainParams), C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint8,
+                                 C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8, C.c_void_p * 3]),
     "oo_cdef_run_units": (None, [_P(abi.CdefElectInput), _P(abi.CdefParams), C.c_int, _P(C.c_uint64)]),
 }
 

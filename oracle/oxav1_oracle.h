@@ -85,6 +85,17 @@ int oo_superres_upscale_recon(const uint16_t *const planes[3], uint32_t width, u
                               uint32_t upscaled_width, uint32_t mi_cols, uint8_t bit_depth, uint8_t ssx, uint8_t ssy,
                               uint8_t num_planes, uint16_t *const out[3]);
 
+/* ---- encoder film-grain election helpers (encoder/film_grain_elect.rs), oo_elect.c ---- */
+void oo_fg_denoise(const uint16_t *const src[3], uint32_t width, uint32_t height, uint8_t ssx, uint8_t ssy,
+                   uint8_t num_planes, uint16_t *const dst[3]);
+void oo_fg_noise_estimate(const uint16_t *const input[3], const uint16_t *const denoised[3], uint32_t width,
+                          uint32_t height, uint8_t ssx, uint8_t ssy, uint8_t num_planes, uint8_t bit_depth,
+                          oxav1_noise_estimate *out);
+uint64_t oo_fg_plane_sigma16(const uint16_t *a, const uint16_t *b, size_t n, uint8_t bit_depth);
+void oo_fg_apply_grain(const oxav1_film_grain_params *fg, const uint16_t *const planes[3], uint32_t width,
+                       uint32_t height, uint8_t bit_depth, uint8_t ssx, uint8_t ssy, uint8_t num_planes,
+                       uint8_t matrix_coefficients, uint16_t *const out[3]);
+
 /* ---- §7.16 superres (superres.rs) ---- */
 int oo_upscale_frame(const oxav1_sr_ctx *ctx, const oxav1_plane_i32 *in, uint32_t nin,
                      oxav1_plane_i32 *out, uint32_t nout);

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -1336,6 +1337,7 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_cdef_ssd_frame") return sizeof(oxav1_cdef_ssd_frame);
     if (n == "oxav1_cdef_elect_input") return sizeof(oxav1_cdef_elect_input);
     if (n == "oxav1_cdef_election") return sizeof(oxav1_cdef_election);
+    if (n == "oxav1_noise_estimate") return sizeof(oxav1_noise_estimate);
     return 0;
 }
 

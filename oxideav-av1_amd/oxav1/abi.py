@@ -217,6 +217,11 @@ class CdefElection(C.Structure):
     ]
 
 
+class NoiseEstimate(C.Structure):
+    _fields_ = [("sigma16", u64), ("bin_sigma16", u64 * 8), ("cb_sigma16", u64), ("cb_bin_sigma16", u64 * 4),
+                ("cr_sigma16", u64), ("cr_bin_sigma16", u64 * 4)]
+
+
 ABI_STRUCTS = {
     "oxav1_plane_i32": Plane, "oxav1_lf_ctx": LfCtx, "oxav1_lf_grids": LfGrids,
     "oxav1_cdef_params": CdefParams, "oxav1_cdef_ctx": CdefCtx, "oxav1_cdef_grids": CdefGrids,
@@ -226,6 +231,7 @@ ABI_STRUCTS = {
     "oxav1_tuning": Tuning, "oxav1_walker_grids": WalkerGrids, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
     "oxav1_server_stats": ServerStats, "oxav1_cdef_cand": CdefCand, "oxav1_cdef_ssd_frame": CdefSsdFrame,
     "oxav1_cdef_elect_input": CdefElectInput, "oxav1_cdef_election": CdefElection,
+    "oxav1_noise_estimate": NoiseEstimate,
 }
 
 _P = C.POINTER
@@ -280,6 +286,12 @@ _SIGS = {
     "oxav1_pf_superres_downscale": (C.c_int, [C.c_void_p * 3, u32, u32, u8, u8, u8, u32, C.c_void_p * 3]),
     "oxav1_pf_superres_upscale_recon": (
         C.c_int, [C.c_void_p * 3, u32, u32, u32, u32, u8, u8, u8, u8, C.c_void_p * 3]),
+    "oxav1_pf_fg_denoise": (C.c_int, [C.c_void_p * 3, u32, u32, u8, u8, u8, C.c_void_p * 3]),
+    "oxav1_pf_fg_noise_estimate": (
+        C.c_int, [C.c_void_p * 3, C.c_void_p * 3, u32, u32, u8, u8, u8, u8, _P(NoiseEstimate)]),
+    "oxav1_pf_fg_plane_sigma16": (C.c_int, [C.c_void_p, C.c_void_p, u64, u8, _P(u64)]),
+    "oxav1_pf_fg_apply_grain": (
+        C.c_int, [_P(FilmGrainParams), C.c_void_p * 3, u32, u32, u8, u8, u8, u8, u8, C.c_void_p * 3]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
     "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -207,3 +207,60 @@ def upscale_recon(planes, upscaled_width, mi_cols, bit_depth, ssx, ssy, oracle=N
     if st:
         raise abi.Oxav1Error(st, "superres_upscale_recon") if oracle is None else RuntimeError(st)
     return out
+
+
+# --- film-grain election helpers (encoder/film_grain_elect.rs) ---------------
+
+def fg_denoise(planes, ssx, ssy, oracle=None):
+    """denoise_frame (film_grain_elect.rs:52-94): u16 planes -> u16 planes."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    h, w = planes[0].shape
+    out = [np.zeros_like(p) for p in planes]
+    st = _fn(oracle, "fg_denoise")(_ptrs(planes), w, h, ssx, ssy, len(planes), _ptrs(out))
+    if oracle is None:
+        abi.check(st, "fg_denoise")
+    return out
+
+
+def _estimate_dict(e):
+    return {"sigma16": e.sigma16, "bin_sigma16": list(e.bin_sigma16), "cb_sigma16": e.cb_sigma16,
+            "cb_bin_sigma16": list(e.cb_bin_sigma16), "cr_sigma16": e.cr_sigma16,
+            "cr_bin_sigma16": list(e.cr_bin_sigma16)}
+
+
+def fg_noise_estimate(planes, denoised, ssx, ssy, bit_depth, oracle=None):
+    """noise_estimate (film_grain_elect.rs:97-167) as a dict of its fields."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    denoised = [np.ascontiguousarray(p, dtype=np.uint16) for p in denoised]
+    h, w = planes[0].shape
+    e = abi.NoiseEstimate()
+    st = _fn(oracle, "fg_noise_estimate")(_ptrs(planes), _ptrs(denoised), w, h, ssx, ssy, len(planes), bit_depth,
+                                          C.byref(e))
+    if oracle is None:
+        abi.check(st, "fg_noise_estimate")
+    return _estimate_dict(e)
+
+
+def fg_plane_sigma16(a, b, bit_depth, oracle=None):
+    """plane_sigma16 (film_grain_elect.rs:896-910)."""
+    a = np.ascontiguousarray(a, dtype=np.uint16).ravel()
+    b = np.ascontiguousarray(b, dtype=np.uint16).ravel()
+    if oracle is not None:
+        return int(oracle.oo_fg_plane_sigma16(a.ctypes.data, b.ctypes.data, a.size, bit_depth))
+    r = C.c_uint64()
+    abi.check(abi.load().oxav1_pf_fg_plane_sigma16(a.ctypes.data, b.ctypes.data, a.size, bit_depth, C.byref(r)),
+              "fg_plane_sigma16")
+    return int(r.value)
+
+
+def fg_apply_grain(fg, planes, bit_depth, ssx, ssy, matrix_coefficients=2, oracle=None):
+    """apply_grain_to_recon (film_grain_elect.rs:834-893): the §7.18.3
+    synthesis of u16 planes with FilmGrainParams `fg`."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+    h, w = planes[0].shape
+    out = [np.zeros_like(p) for p in planes]
+    st = _fn(oracle, "fg_apply_grain")(C.byref(fg), _ptrs(planes), w, h, bit_depth, ssx, ssy, len(planes),
+                                       matrix_coefficients, _ptrs(out))
+    if oracle is None:
+        abi.check(st, "fg_apply_grain")
+    return out

@@ -191,3 +191,42 @@ def test_superres_helpers_match_oracle(oracle_mod, fmt):
             up = E.upscale_recon(rec, w, (cw + 3) // 4, bd, sx, sy)
             up_o = E.upscale_recon(rec, w, (cw + 3) // 4, bd, sx, sy, oo)
             assert all(np.array_equal(a, b) for a, b in zip(up, up_o)), d
+
+
+# --- film-grain election helpers (encoder/film_grain_elect.rs) ---------------
+
+FG_FORMATS = [  # (w, h, bit depth, ssx, ssy, mono)
+    (150, 98, 8, 1, 1, False), (128, 70, 10, 1, 0, False), (96, 64, 12, 0, 0, False), (120, 66, 8, 1, 1, True),
+    (1920, 1080, 10, 1, 1, False),
+]
+
+
+@pytest.mark.parametrize("fmt", FG_FORMATS)
+def test_fg_helpers_match_oracle(oracle_mod, fmt):
+    from conftest import gen_config
+    from oxav1 import Frame
+    oo = oracle_mod.load()
+    w, h, bd, sx, sy, mono = fmt
+    print(f"film-grain helpers {fmt}", flush=True)
+    rng = np.random.default_rng(w * h)
+    pl = _planes(rng, w, h, bd, sx, sy, mono, True)
+    pl = [np.clip(p.astype(np.int64) + rng.normal(0, 3 << (bd - 8), p.shape), 0, (1 << bd) - 1).astype(np.uint16)
+          for p in pl]
+    den = E.fg_denoise(pl, sx, sy)
+    assert all(np.array_equal(a, b) for a, b in zip(den, E.fg_denoise(pl, sx, sy, oo)))
+    assert E.fg_noise_estimate(pl, den, sx, sy, bd) == E.fg_noise_estimate(pl, den, sx, sy, bd, oo)
+    for p in range(len(pl)):
+        assert E.fg_plane_sigma16(pl[p], den[p], bd) == E.fg_plane_sigma16(pl[p], den[p], bd, oo)
+    # grain on the reconstruction: planes of ((w + sx) >> sx) x ((h + sy) >> sy)
+    f = Frame.generate(gen_config(enable_fg=1, fg_num_y_points=8, fg_ar_lag=int(rng.integers(0, 4)),
+                                  fg_chroma_points=0 if mono else 4, fg_overlap=int(rng.integers(0, 2)),
+                                  bit_depth=bd, mono=int(mono), subsampling_x=sx, subsampling_y=sy,
+                                  noise_amp=8), 0xF6B0 + w)
+    rec = [rng.integers(0, 1 << bd, (h, w), dtype=np.uint16)]
+    if not mono:
+        rec += [rng.integers(0, 1 << bd, ((h + sy) >> sy, (w + sx) >> sx), dtype=np.uint16) for _ in range(2)]
+    fg = f.desc.fg
+    got = E.fg_apply_grain(fg, rec, bd, sx, sy, f.desc.matrix_coefficients)
+    want = E.fg_apply_grain(fg, rec, bd, sx, sy, f.desc.matrix_coefficients, oo)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert any(not np.array_equal(a, b) for a, b in zip(got, rec))

@@ -285,3 +285,92 @@ def test_upscale_recon_is_upscale_frame(oo, oracle_mod):
         assert oo.oo_upscale_frame(C.byref(ctx), planes_struct(src), 3, planes_struct(dst), 3) == 0
         for g, d in zip(got, dst):
             assert np.array_equal(g, d)
+
+
+# --- film-grain election helpers (encoder/film_grain_elect.rs) ---------------
+
+def _np_denoise(p):
+    a = np.pad(p.astype(np.int64), 1, mode="edge")
+    k = np.array([[1, 2, 1], [2, 4, 2], [1, 2, 1]])
+    s = sum(k[i, j] * a[i:i + p.shape[0], j:j + p.shape[1]] for i in range(3) for j in range(3))
+    return ((s + 8) >> 4).astype(np.uint16)
+
+
+def test_fg_denoise(oo):
+    # film_grain_elect.rs:957-964: flat fields stay exact
+    f = [np.full((16, 32), 173, np.uint16), np.full((8, 16), 173, np.uint16), np.full((8, 16), 173, np.uint16)]
+    d = E.fg_denoise(f, 1, 1, oo)
+    assert all(np.array_equal(a, b) for a, b in zip(d, f))
+    # a planar ramp is a fixed point of the binomial away from the edges
+    y, x = np.mgrid[0:20, 0:30]
+    ramp = (3 * x + 5 * y + 11).astype(np.uint16)
+    assert np.array_equal(E.fg_denoise([ramp], 0, 0, oo)[0][1:-1, 1:-1], ramp[1:-1, 1:-1])
+    # the binomial with edge replication, as a numpy convolution
+    rng = np.random.default_rng(3)
+    for sx, sy in ((1, 1), (1, 0), (0, 0)):
+        pl = [rng.integers(0, 4096, (18, 26), dtype=np.uint16)]
+        pl += [rng.integers(0, 4096, (18 >> sy, 26 >> sx), dtype=np.uint16) for _ in range(2)]
+        for a, b in zip(E.fg_denoise(pl, sx, sy, oo), pl):
+            assert np.array_equal(a, _np_denoise(b))
+
+
+def _np_sigma16(sq, n):
+    return 0 if n == 0 else int(np.round(np.sqrt(sq / n) * 16.0))
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_fg_noise_estimate_and_plane_sigma(oo, bd):
+    rng = np.random.default_rng(bd)
+    sh = bd - 8
+    # a constant offset of k steps: every sigma is 16 |k|
+    den = [rng.integers(0, 1 << bd, (40, 64), dtype=np.uint16)] + \
+          [rng.integers(0, 1 << bd, (20, 32), dtype=np.uint16) for _ in range(2)]
+    k = 3
+    inp = [np.clip(d.astype(np.int64) + (k << sh), 0, (1 << bd) - 1).astype(np.uint16) for d in den]
+    den = [(i.astype(np.int64) - (k << sh)).astype(np.uint16) for i in inp]  # exact offsets after the clip
+    e = E.fg_noise_estimate(inp, den, 1, 1, bd, oo)
+    assert e["sigma16"] == 16 * k and e["cb_sigma16"] == 16 * k and e["cr_sigma16"] == 16 * k
+    assert all(v == 16 * k for v in e["bin_sigma16"] + e["cb_bin_sigma16"] + e["cr_bin_sigma16"])
+    # random noise: the same statistics in numpy (bins of 32 / 64 on the
+    # 8-bit scale, sparse bins fall back to the plane's sigma)
+    inp = [np.clip(d.astype(np.int64) + rng.normal(0, 6 << sh, d.shape), 0, (1 << bd) - 1).astype(np.uint16)
+           for d in den]
+    inp[0][:, :8] = 0  # a well-populated darkest bin; several bins stay sparse
+    e = E.fg_noise_estimate(inp, den, 1, 1, bd, oo)
+
+    def profile(a, b, nb, bs, floor):
+        d = (a.astype(np.int64) - b.astype(np.int64)) >> sh
+        sq = d * d
+        n = a.size
+        g = _np_sigma16(sq.sum(), n)
+        bins = np.minimum((a.astype(np.int64) >> sh) >> bs, nb - 1)
+        out = []
+        for i in range(nb):
+            m = bins == i
+            out.append(_np_sigma16(sq[m].sum(), m.sum()) if m.sum() >= max(n // 100, floor) else g)
+        return g, out
+    g, bins = profile(inp[0], den[0], 8, 5, 64)
+    assert e["sigma16"] == g and e["bin_sigma16"] == bins
+    for p, key in ((1, "cb"), (2, "cr")):
+        g, bins = profile(inp[p], den[p], 4, 6, 32)
+        assert e[key + "_sigma16"] == g and e[key + "_bin_sigma16"] == bins
+    for p in range(3):
+        d = (inp[p].astype(np.int64) - den[p].astype(np.int64)) >> sh
+        assert E.fg_plane_sigma16(inp[p], den[p], bd, oo) == _np_sigma16((d * d).sum(), d.size)
+    assert E.fg_plane_sigma16(inp[0][:0], den[0][:0], bd, oo) == 0
+
+
+def test_fg_apply_grain_is_synthesis(oo, oracle_mod):
+    """apply_grain_to_recon is §7.18.3 (pinned by the film-grain tests) on
+    the u16 planes with the shapes of film_grain_elect.rs:847-856."""
+    from conftest import gen_config
+    from oxav1 import Frame
+    f = Frame.generate(gen_config(enable_fg=1, fg_num_y_points=8, fg_ar_lag=2, fg_chroma_points=4, fg_overlap=1,
+                                  bit_depth=10, noise_amp=32), 0xF6A1)
+    d = f.desc
+    planes = [f.planes[p][: f.out_shape(p)[0], : f.out_shape(p)[1]].astype(np.uint16) for p in range(3)]
+    got = E.fg_apply_grain(d.fg, planes, 10, d.subsampling_x, d.subsampling_y, d.matrix_coefficients, oo)
+    want = oracle_mod.calls().film_grain_synthesis(d.fg, 10, 3, d.subsampling_x, d.subsampling_y,
+                                                   d.matrix_coefficients, [p.astype(np.int32) for p in planes])
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert any(not np.array_equal(a, b) for a, b in zip(got, planes))
