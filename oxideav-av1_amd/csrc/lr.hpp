@@ -31,6 +31,18 @@ constexpr int kLrSh = kLrMaxH + 6;          // source window height
 constexpr int kLrAw = kLrMaxW + 2;          // A/B columns: tile cols -1 .. W
 constexpr int kLrAh1 = kLrMaxH + 2;         // pass-1 A/B rows: tile rows -1 .. H
 constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // pass-0 A/B rows: the odd rows only
+// Output rows per wave when four waves share a chunk, and A/B rows per
+// thread band when at least three bands of (tile width + 2) columns fit in a
+// workgroup: the row loops are unrolled to these bounds, so their register
+// windows need no rotating copies.
+constexpr int kLrBand = (kLrMaxH + 3) / 4;
+constexpr int kLrAbBand = (kLrMaxH + 2 + 2) / 3;
+// Source-window rows per wave with four waves (the window is kLrSh rows).
+constexpr int kLrWinRows = (kLrSh + 3) / 4;
+
+// x * y for |x|, |y| < 2^23: the low 32 bits of the product (the same bits
+// a wrapping 32-bit multiply gives) on the full-rate 24-bit multiplier.
+__device__ __forceinline__ int32_t lr_mul24(int32_t x, int32_t y) { return __mul24(x, y); }
 
 __constant__ int32_t c_sgr_params[16][4];   // filled from kSgrParams at init
 __constant__ uint16_t c_sgr_a2[256];        // a2 for 0 < z < 255 (loop_restoration.rs:648-654)
@@ -138,9 +150,18 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     const int b0 = wv * RB, b1 = min(H, b0 + RB);
     if (type != OXAV1_RESTORE_WIENER && type != OXAV1_RESTORE_SGRPROJ) {
         if (!J.lr_copy_none) return;
+        // every load of the band is issued before the first store, so the
+        // copy waits for memory once, not once per row
         for (int x = lane; x < W; x += 64)
-            for (int r = b0; r < b1; r++)
-                out[(size_t)(g.ys + r) * opitch + (g.xs + x)] = cdf[(size_t)(g.ys + r) * dpitch + (g.xs + x)];
+            for (int rb = b0; rb < b1; rb += kLrBand) {
+                T v[kLrBand];
+#pragma unroll
+                for (int i = 0; i < kLrBand; i++)
+                    v[i] = cdf[(size_t)(g.ys + min(rb + i, b1 - 1)) * dpitch + (g.xs + x)];
+#pragma unroll
+                for (int i = 0; i < kLrBand; i++)
+                    if (rb + i < b1) out[(size_t)(g.ys + rb + i) * opitch + (g.xs + x)] = v[i];
+            }
         return;
     }
     // Routed source window (get_source_sample, loop_restoration.rs:862-902):
@@ -178,20 +199,20 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     __syncthreads();
     {
         // Window rows go to the waves in turn, lanes to columns 0..63 and
-        // 64..ww-1, so the column clamps are computed once per lane.  Four
-        // rows' loads are issued before their stores (their latencies
-        // overlap); every load is in bounds: rows past the window repeat the
-        // last row (no store), rows without a buffer read the CDEF plane's
-        // first sample and store 0.
+        // 64..ww-1, so the column clamps are computed once per lane.  All of
+        // a wave's rows (at most kLrWinRows) are loaded before the first
+        // store, so the window costs one memory round trip; every load is in
+        // bounds: rows past the window repeat the last row (no store), rows
+        // without a buffer read the CDEF plane's first sample and store 0.
         const int32_t xmax = max(g.pex, 0);
         const int32_t xa = clip3(0, xmax, g.wx0 + lane);
         const int32_t xb = clip3(0, xmax, g.wx0 + lane + 64);
         const bool st_a = lane < g.ww, st_b = lane + 64 < g.ww;
         const int nrw = (g.wh - wv + nwv - 1) / nwv;  // this wave's rows (uniform in the wave)
-        for (int i0 = 0; i0 < nrw; i0 += 4) {
-            L va[4], vb[4];
+        for (int i0 = 0; i0 < nrw; i0 += kLrWinRows) {
+            L va[kLrWinRows], vb[kLrWinRows];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < kLrWinRows; j++) {
                 const int rr = min(wv + (i0 + j) * nwv, g.wh - 1);
                 const int32_t cm = S.rowmax[rr];
                 const G<T> *row = cm < 0 ? cdf : (const G<T> *)S.rowp[rr];
@@ -200,7 +221,7 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
                 vb[j] = cm < 0 ? (L)0 : (L)row[min(xb, c)];
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < kLrWinRows; j++) {
                 const int r = wv + (i0 + j) * nwv;
                 if (i0 + j < nrw) {
                     if (st_a) S.src[r * kLrSw + lane] = va[j];
@@ -240,16 +261,23 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
         }
         __syncthreads();
         if (lane >= W || b0 >= b1) return;
-        // vertical pass down the wave's band with a 7-row register window
+        // vertical pass down the wave's band, kLrBand rows at a time from
+        // registers.  The horizontal results are clipped to
+        // [-offset, limit - offset] (|h| < 2^15) and |vf| < 2^18 for any
+        // int16 taps, so the 24-bit multiplies give the 32-bit products.
         const int32_t *iv = S.inter + lane;
-        int32_t h0 = iv[b0 * kLrMaxW], h1 = iv[(b0 + 1) * kLrMaxW], h2 = iv[(b0 + 2) * kLrMaxW],
-                h3 = iv[(b0 + 3) * kLrMaxW], h4 = iv[(b0 + 4) * kLrMaxW], h5 = iv[(b0 + 5) * kLrMaxW];
-        G<T> *o = out + (size_t)(g.ys + b0) * opitch + (g.xs + lane);
-        for (int r = b0; r < b1; r++, o += opitch) {
-            const int32_t h6 = iv[(r + 6) * kLrMaxW];
-            const int32_t acc = vf[0] * (h0 + h6) + vf[1] * (h1 + h5) + vf[2] * (h2 + h4) + vf[3] * h3;
-            *o = (T)clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1);
-            h0 = h1, h1 = h2, h2 = h3, h3 = h4, h4 = h5, h5 = h6;
+        for (int rb = b0; rb < b1; rb += kLrBand) {
+            int32_t h[kLrBand + 6];
+#pragma unroll
+            for (int i = 0; i < kLrBand + 6; i++) h[i] = rb + i < H + 6 ? iv[(rb + i) * kLrMaxW] : 0;
+            G<T> *o = out + (size_t)(g.ys + rb) * opitch + (g.xs + lane);
+#pragma unroll
+            for (int i = 0; i < kLrBand; i++)
+                if (rb + i < b1) {
+                    const int32_t acc = lr_mul24(vf[0], h[i] + h[i + 6]) + lr_mul24(vf[1], h[i + 1] + h[i + 5]) +
+                                        lr_mul24(vf[2], h[i + 2] + h[i + 4]) + lr_mul24(vf[3], h[i + 3]);
+                    o[(size_t)i * opitch] = (T)clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1);
+                }
         }
         return;
     }
@@ -286,36 +314,44 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
                 h5[slot] = h3[slot] + v0 + v4;
                 q5[slot] = q3[slot] + v0 * v0 + v4 * v4;
             };
-            // rows k0-2 .. k0+1 preloaded into slots 0..3
-#pragma unroll
-            for (int j = 0; j < 4; j++) hsum(k0 - 2 + j + 3, j);
+            // The band is walked kLrAbBand rows at a time (one step when
+            // three or more bands fit, W + 2 <= 85), each step unrolled so
+            // the five-row windows are renamed, not copied: rows kb-2 ..
+            // kb+1 preloaded into slots 0..3, then one new row per output row.
             const int32_t ys = g.ys;
-            for (int k = k0; k <= k1; k++) {
-                hsum(k + 2 + 3, 4);
-                if (r1) {
-                    uint32_t a;
-                    int32_t b;
-                    if (r1 == 1) {
-                        a = (uint32_t)(q3[1] + q3[2] + q3[3]);
-                        b = h3[1] + h3[2] + h3[3];
-                    } else {
-                        a = (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
-                        b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
-                    }
-                    S.ab1[(k + 1) * kLrAw + c] = lr_ab(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
-                }
-                if (r0 && ((ys + k) & 1)) {
-                    const uint32_t a =
-                        (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
-                    const int32_t b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
-                    S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
-                }
+            for (int kb = k0; kb <= k1; kb += kLrAbBand) {
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    h3[j] = h3[j + 1];
-                    q3[j] = q3[j + 1];
-                    h5[j] = h5[j + 1];
-                    q5[j] = q5[j + 1];
+                for (int j = 0; j < 4; j++) hsum(kb - 2 + j + 3, j);
+#pragma unroll
+                for (int i = 0; i < kLrAbBand; i++) {
+                    const int k = kb + i;
+                    if (k > k1) break;
+                    hsum(k + 2 + 3, 4);
+                    if (r1) {
+                        uint32_t a;
+                        int32_t b;
+                        if (r1 == 1) {
+                            a = (uint32_t)(q3[1] + q3[2] + q3[3]);
+                            b = h3[1] + h3[2] + h3[3];
+                        } else {
+                            a = (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
+                            b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
+                        }
+                        S.ab1[(k + 1) * kLrAw + c] = lr_ab(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
+                    }
+                    if (r0 && ((ys + k) & 1)) {
+                        const uint32_t a =
+                            (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
+                        const int32_t b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
+                        S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        h3[j] = h3[j + 1];
+                        q3[j] = q3[j + 1];
+                        h5[j] = h5[j + 1];
+                        q5[j] = q5[j + 1];
+                    }
                 }
             }
         }
@@ -352,46 +388,56 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
         const int kp = ((ys + b0) & 1) ? b0 : b0 - 1;
         prow(kp, pa_prev, pb_prev);
     }
-    G<T> *o = out + (size_t)(ys + b0) * opitch + (g.xs + lane);
-    for (int k = b0; k < b1; k++, o += opitch) {
-        const int32_t uval = (int32_t)S.src[(k + 3) * kLrSw + lane + 3];
-        const int32_t u4 = uval << 4;
-        int32_t f0 = u4, f1 = u4;
-        if (r0) {
-            int32_t as, bs;
-            uint32_t sh;
-            if ((ys + k) & 1) {  // P(k) is already held: set at b0 or by row k-1
-                as = pa_prev;
-                bs = pb_prev;
-                sh = 4;
-            } else {
-                int32_t pa_next, pb_next;
-                prow(k + 1, pa_next, pb_next);
-                as = pa_prev + pa_next;
-                bs = pb_prev + pb_next;
-                pa_prev = pa_next;
-                pb_prev = pb_next;
-                sh = 5;
+    // With 16-bit samples every product below has both operands under 2^23
+    // (A sums <= 8192, samples < 2^16, filter outputs < 2^22, weights from
+    // int16 xqd), so it runs on the 24-bit multiplier.
+    constexpr bool k24 = std::is_same<L, uint16_t>::value;
+    auto mul = [](int32_t x, int32_t y) -> int32_t { return k24 ? lr_mul24(x, y) : x * y; };
+    for (int kb = b0; kb < b1; kb += kLrBand) {
+        G<T> *o = out + (size_t)(ys + kb) * opitch + (g.xs + lane);
+#pragma unroll
+        for (int i = 0; i < kLrBand; i++) {
+            const int k = kb + i;
+            if (k >= b1) break;
+            const int32_t uval = (int32_t)S.src[(k + 3) * kLrSw + lane + 3];
+            const int32_t u4 = uval << 4;
+            int32_t f0 = u4, f1 = u4;
+            if (r0) {
+                int32_t as, bs;
+                uint32_t sh;
+                if ((ys + k) & 1) {  // P(k) is already held: set at b0 or by row k-1
+                    as = pa_prev;
+                    bs = pb_prev;
+                    sh = 4;
+                } else {
+                    int32_t pa_next, pb_next;
+                    prow(k + 1, pa_next, pb_next);
+                    as = pa_prev + pa_next;
+                    bs = pb_prev + pb_next;
+                    pa_prev = pa_next;
+                    pb_prev = pb_next;
+                    sh = 5;
+                }
+                const int32_t v = mul(as, uval) + bs;
+                const uint32_t ts = 8 + sh - 4;
+                f0 = (v + (1 << (ts - 1))) >> ts;
             }
-            const int32_t v = as * uval + bs;
-            const uint32_t ts = 8 + sh - 4;
-            f0 = (v + (1 << (ts - 1))) >> ts;
+            if (r1) {
+                int32_t sa2, ma2, sb2, mb2;
+                row3(S.ab1, (k + 2) * kLrAw, sa2, ma2, sb2, mb2);
+                const int32_t as = 3 * (sa0 + sa2) + ma0 + ma2 + 4 * sa1;
+                const int32_t bs = 3 * (sb0 + sb2) + mb0 + mb2 + 4 * sb1;
+                const int32_t v = mul(as, uval) + bs;
+                f1 = (v + (1 << 8)) >> 9;
+                sa0 = sa1, ma0 = ma1, sb0 = sb1, mb0 = mb1;
+                sa1 = sa2, ma1 = ma2, sb1 = sb2, mb1 = mb2;
+            }
+            int32_t v = mul(w1, u4);
+            v += mul(w0, r0 != 0 ? f0 : u4);
+            v += mul(w2, r1 != 0 ? f1 : u4);
+            const int32_t sres = (v + (1 << 10)) >> 11;
+            o[(size_t)i * opitch] = (T)clip3(0, maxv, sres);
         }
-        if (r1) {
-            int32_t sa2, ma2, sb2, mb2;
-            row3(S.ab1, (k + 2) * kLrAw, sa2, ma2, sb2, mb2);
-            const int32_t as = 3 * (sa0 + sa2) + ma0 + ma2 + 4 * sa1;
-            const int32_t bs = 3 * (sb0 + sb2) + mb0 + mb2 + 4 * sb1;
-            const int32_t v = as * uval + bs;
-            f1 = (v + (1 << 8)) >> 9;
-            sa0 = sa1, ma0 = ma1, sb0 = sb1, mb0 = mb1;
-            sa1 = sa2, ma1 = ma2, sb1 = sb2, mb1 = mb2;
-        }
-        int32_t v = w1 * u4;
-        v += r0 != 0 ? w0 * f0 : w0 * u4;
-        v += r1 != 0 ? w2 * f1 : w2 * u4;
-        const int32_t sres = (v + (1 << 10)) >> 11;
-        *o = (T)clip3(0, maxv, sres);
     }
 }
 
