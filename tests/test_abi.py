@@ -64,3 +64,50 @@ def test_generator_geometry():
     assert f.desc.mi_cols == 2 * ((100 + 7) >> 3) and f.desc.mi_rows == 2 * ((75 + 7) >> 3)
     assert f.planes[0].shape == (f.desc.mi_rows * 4, f.desc.mi_cols * 4)
     assert f.planes[1].shape == (f.desc.mi_rows * 2, f.desc.mi_cols * 2)
+
+
+def test_election_entry_points_reject_bad_arguments_before_the_device():
+    """Argument checks of the election entry points return their status
+    before any device work (so they run without a GPU)."""
+    from oxav1 import abi
+    lib = abi.load()
+    cand = (abi.CdefCand * 129)()
+    f = abi.CdefSsdFrame()
+    # more candidates than one launch holds
+    assert lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, cand, 129, None) == abi.ERR_INVALID_ARG
+    # a frame without grids or tables
+    assert lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, cand, 1, None) == abi.ERR_INVALID_ARG
+    # planes the 8x8 walk does not cover
+    f.mi_rows = f.mi_cols = 2
+    f.cdef_idx = f.skip = f.ssd = 16
+    f.recon[0] = f.input[0] = 16
+    f.rows[0], f.cols[0] = 8, 16
+    f.recon_pitch[0] = f.input_pitch[0] = 16
+    f.bit_depth, f.num_planes, f.sample_bytes = 8, 1, 1
+    cand[0] = abi.CdefCand(3, 0, 1, 0)
+    assert lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, cand, 1, None) == abi.ERR_UNSUPPORTED
+    # candidate outside the legal strengths / dampings
+    f.cols[0] = 8
+    cand[0] = abi.CdefCand(2, 0, 1, 0)
+    assert lib.oxav1_pf_cdef_unit_ssd(C.byref(f), 1, cand, 1, None) == abi.ERR_INVALID_ARG
+    # elections without input
+    assert lib.oxav1_pf_cdef_elect(None, None) == abi.ERR_INVALID_ARG
+    inp, out = abi.CdefElectInput(), abi.CdefElection()
+    assert lib.oxav1_pf_cdef_elect(C.byref(inp), C.byref(out)) == abi.ERR_INVALID_ARG
+    # superres / film-grain helpers
+    p = (C.c_void_p * 3)(16, 16, 16)
+    assert lib.oxav1_pf_superres_downscale(p, 64, 8, 1, 1, 3, 64, p) == abi.ERR_INVALID_ARG  # not narrower
+    assert lib.oxav1_pf_superres_downscale(p, 64, 8, 1, 1, 2, 32, p) == abi.ERR_INVALID_ARG  # two planes
+    r = C.c_int32(7)
+    assert lib.oxav1_pf_superres_probe(None, 0, 0, 9, C.byref(r)) == abi.ERR_INVALID_ARG  # bit depth 9
+    assert lib.oxav1_pf_superres_probe(None, 2, 4, 8, C.byref(r)) == abi.ERR_INVALID_ARG
+    assert lib.oxav1_pf_superres_upscale_recon(p, 64, 8, 96, 16, 7, 1, 1, 3, p) == abi.ERR_INVALID_ARG
+    s = C.c_uint64(5)
+    assert lib.oxav1_pf_fg_plane_sigma16(None, None, 0, 8, C.byref(s)) == 0 and s.value == 0
+    e = abi.NoiseEstimate()
+    assert lib.oxav1_pf_fg_noise_estimate(p, p, 8, 8, 1, 1, 3, 11, C.byref(e)) == abi.ERR_INVALID_ARG
+    assert lib.oxav1_pf_fg_apply_grain(None, p, 8, 8, 8, 1, 1, 3, 0, p) == abi.ERR_INVALID_ARG
+    # host-only helpers
+    assert lib.oxav1_pf_superres_coded_width(3840, 12) == 2560
+    assert lib.oxav1_pf_superres_coded_width(3840, 0) == 0
+    assert lib.oxav1_pf_superres_candidate_denoms(3840, None) > 0
