@@ -393,7 +393,9 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
     G<T> *out = plane_ptr<T>(J.out[plane]);
     const uint32_t ip = J.ref_pitch[plane], op = J.out_pitch[plane];
     const int32_t w = (int32_t)J.out_cols[0];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform in a wave: a scalar, so the band's rows and
+    // their base addresses are scalar too
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t x = x0 + lane;
     const int32_t yb0 = y0 + 16 * wv, yb1 = min(y1, yb0 + 16);
     if (x >= x1 || yb0 >= yb1) return;

@@ -125,7 +125,9 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     g.xe = min(g.xe, (int32_t)J.out_cols[plane] - 1);
     if (g.ys > g.ye || g.xs > g.xe) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wv = tid >> 6, nwv = nt >> 6;
+    // the wave index is uniform in a wave: a scalar, so row bands and their
+    // base addresses are scalar too
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), nwv = nt >> 6;
     const int32_t W = g.xe - g.xs + 1;
     const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
     const uint32_t dpitch = J.up_cdef_pitch[plane];
