@@ -187,7 +187,11 @@ __device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int
     if (xx >= tw) return;
     const int32_t px = ox + xx;
     if (px >= pw) return;
-    for (int yy = threadIdx.x >> row_log2; yy < th; yy += ystep) {
+    // first row: the wave's (scalar) base plus the lane's sub-row when a
+    // wave covers several rows
+    const int y_first = (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) << (6 - row_log2)) +
+                        ((int)(threadIdx.x & 63) >> row_log2);
+    for (int yy = y_first; yy < th; yy += ystep) {
         const int32_t py = oy + yy;
         if (py >= ph) break;
         const CdefTaps b = taps[(yy >> bh_log2) * 8 + (xx >> bw_log2)];
