@@ -121,12 +121,16 @@ __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSme
 }
 
 template <typename T>
-__device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut) {
+__device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut, const void *&lut_of) {
     const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
     const uint32_t t = local - J.item_base[kStageFg][p];
     const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
-    if (J.apply_fg)  // jobs without grain have no grain scratch and only copy
+    // the frame's scaling LUTs, staged again only when the frame changes
+    // (jobs without grain have no grain scratch and only copy)
+    if (J.apply_fg && J.fgs != lut_of) {
         for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)J.fgs)->lut[i >> 8][i & 255];
+        lut_of = J.fgs;
+    }
     __syncthreads();
     const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
     const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
@@ -252,14 +256,15 @@ template <int KIND>
 __device__ __forceinline__ void fg_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter, uint32_t claim) {
     __shared__ uint8_t lut[768];
+    const void *lut_of = nullptr;  // whose LUTs lut holds (uniform in the workgroup)
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            fg_item<int32_t>(J, ref.local, lut);
+            fg_item<int32_t>(J, ref.local, lut, lut_of);
         } else {
             if (J.kind == kU8)
-                fg_item<uint8_t>(J, ref.local, lut);
+                fg_item<uint8_t>(J, ref.local, lut, lut_of);
             else
-                fg_item<uint16_t>(J, ref.local, lut);
+                fg_item<uint16_t>(J, ref.local, lut, lut_of);
         }
     })
 }
