@@ -143,9 +143,37 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
 // workgroups contend for that one address, so stages with many cheap items
 // claim several at once), claim == 0 walks items blockIdx.x + k * gridDim.x
 // without atomics.
+// A stage's items are split into kClaimParts contiguous partitions, each with
+// its own counter on its own 128-byte line: a workgroup claims from the
+// partition of its XCD (workgroups are dealt to the XCDs round-robin) and,
+// once that is exhausted, from the next ones in turn.  Eight counters take
+// the atomics of one; within a partition consecutive claims stay in one
+// frame and one XCD's L2.  Sets first/count = (total, 0) when every
+// partition is exhausted.
+constexpr uint32_t kClaimParts = 8, kClaimStride = 32;
+__device__ __forceinline__ void claim_items(uint32_t *counter, uint32_t total, uint32_t claim, uint32_t &part,
+                                            uint32_t &first, uint32_t &count) {
+    const uint32_t home = blockIdx.x & (kClaimParts - 1);
+    for (; part < kClaimParts; part++) {
+        const uint32_t r = (home + part) & (kClaimParts - 1);
+        const uint32_t lo = (uint32_t)((uint64_t)total * r / kClaimParts);
+        const uint32_t hi = (uint32_t)((uint64_t)total * (r + 1) / kClaimParts);
+        if (lo >= hi) continue;
+        const uint32_t v = lo + atomicAdd(counter + r * kClaimStride, claim);
+        if (v < hi) {
+            first = v;
+            count = min(claim, hi - v);
+            return;
+        }
+    }
+    first = total;
+    count = 0;
+}
+
 #define OXAV1_ITEM_LOOP(BODY)                                                     \
-    __shared__ uint32_t s_item;                                                   \
+    __shared__ uint32_t s_item, s_n;                                              \
     uint32_t base_ = 0, left_ = 0;                                                \
+    uint32_t part_ = 0; /* partitions tried so far (thread 0) */                  \
     uint32_t cf_ = 0, clo_ = 0, chi_ = 0; /* items [clo_, chi_) are frame cf_ */  \
     for (uint32_t iter = 0;; iter++) {                                            \
         uint32_t item;                                                            \
@@ -156,10 +184,10 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
             item = blockIdx.x + iter * gridDim.x;                                 \
         } else {                                                                  \
             if (left_ == 0) {                                                     \
-                if (threadIdx.x == 0) s_item = atomicAdd(counter, claim);         \
+                if (threadIdx.x == 0) claim_items(counter, total, claim, part_, s_item, s_n); \
                 __syncthreads();                                                  \
                 base_ = s_item;                                                   \
-                left_ = claim;                                                    \
+                left_ = s_n ? s_n : 1u;                                           \
                 __syncthreads();                                                  \
             }                                                                     \
             item = base_++;                                                       \
@@ -345,8 +373,11 @@ __global__ void k_build_grids(oxav1_walker_grids g, uint32_t mi_rows, uint32_t m
 }
 
 // Zero the batched work counters (a kernel, so the reset is graph-safe).
+// Work counters of one engine: five stages x kClaimParts partitions.
+constexpr uint32_t kCounterWords = 5 * kClaimParts * kClaimStride;
+
 __global__ void k_reset(uint32_t *counters, uint32_t n) {
-    if (threadIdx.x < n) counters[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
 }
 
 // ===================================================================== //
@@ -798,7 +829,7 @@ extern "C" int oxav1_pf_engine_create(oxav1_pf_engine **out, void *hip_stream, u
         delete e;
         return OXAV1_ERR_HIP;
     }
-    st = e->counters_d.ensure(8 * sizeof(uint32_t));
+    st = e->counters_d.ensure(kCounterWords * sizeof(uint32_t));
     if (st) {
         delete e;
         return st;
@@ -970,7 +1001,7 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
     const FrameJob *jd = (const FrameJob *)e->jobs_d.p;
     uint32_t *ctr = (uint32_t *)e->counters_d.p;
     const uint32_t *pre = (const uint32_t *)e->prefix_d.p;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, e->stream, ctr, 8u);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, e->stream, ctr, kCounterWords);
     HIP_TRY(hipGetLastError());
     // Grain templates depend only on the parameters: build them on a side
     // branch that overlaps deblock..restoration, joined before the blend.
@@ -994,7 +1025,7 @@ static int enqueue_batch(oxav1_pf_engine *e, bool timed) {
             const uint32_t grid =
                 std::min<uint32_t>(total, (uint32_t)e->num_cus * (per_cu ? std::min(per_cu, fit) : fit));
             hipLaunchKernelGGL(kEngineKernels[s][w], dim3(grid), dim3(kThreads), 0, e->stream, jd, pre + s * (e->n + 1),
-                               e->n, total, ctr + s, claim);
+                               e->n, total, ctr + s * kClaimParts * kClaimStride, claim);
             HIP_TRY(hipGetLastError());
         }
         if (timed) HIP_TRY(hipEventRecord(e->ev[s + 1], e->stream));
