@@ -24,6 +24,13 @@ __constant__ uint8_t c_cdef_uv_dir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2,
                                                {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
 __constant__ int32_t c_div_table[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
 
+// Median of three signed integers in one VALU instruction (v_med3_i32).
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Per 8x8 block decisions shared by the unit's threads.
 struct CdefBlk {
     int8_t dir_y, dir_uv;
@@ -221,7 +228,7 @@ __device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int
                     else
                         ad = (uint32_t)abs(d);
                     const int32_t tt = (int32_t)__builtin_elementwise_sub_sat(thr, ad >> adj);
-                    sum += w * max(min(d, tt), -tt);
+                    sum += w * med3_i32(d, -tt, tt);  // d clamped to [-tt, tt], tt >= 0
                     mx = max(mx, v);
                     mn = min(mn, v);
                 }
