@@ -148,7 +148,13 @@ struct LfRegion {
     }
 };
 
-__device__ __forceinline__ int32_t f4clamp(int32_t v, int32_t lo, int32_t hi) { return clip3(lo, hi, v); }
+// lo = -(1 << (bd - 1)) < hi = (1 << (bd - 1)) - 1: the clamp is the median of
+// three, one v_med3_i32
+__device__ __forceinline__ int32_t f4clamp(int32_t v, int32_t lo, int32_t hi) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+    return r;
+}
 
 // wide_filter (loop_filter.rs:789-822) with N = n, N2 = n2, LOG2 = log2Size
 // fixed at compile time: output i in [-N, N) is
