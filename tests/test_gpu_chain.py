@@ -69,6 +69,26 @@ def test_chain_small_mixed(mode, oracle_mod):
     eng.close()
 
 
+@pytest.mark.parametrize("sizes", [[(16, 16)], [(24, 16), (64, 40), (16, 24)], [(40, 24)] * 11])
+def test_chain_batches_smaller_than_the_claim_partitions(sizes, oracle_mod):
+    """Batched launches whose stages have fewer work items than there are
+    claim partitions (empty partitions, items stolen across partitions)
+    and more frames than partitions: every frame equals the oracle."""
+    from oxav1.engine import Engine
+    frames = _frames([(gen_config(upscaled_width=w, frame_height=h, superres_denom=8 + (i % 3),
+                                  enable_fg=i % 2, fg_num_y_points=4), 0x7A00 + i)
+                      for i, (w, h) in enumerate(sizes)])
+    eng = Engine(max_frames=len(frames))
+    eng.prepare(frames)
+    for _ in range(2):
+        for df in frames:
+            df.clear_outputs()
+        eng.run(2)
+        eng.sync()
+        _check(frames, oracle_mod, f"{len(frames)} small frames")
+    eng.close()
+
+
 @pytest.mark.parametrize("cfg", [0, 1])
 def test_chain_config(cfg, oracle_mod):
     from oxav1 import Frame
