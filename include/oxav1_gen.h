@@ -46,6 +46,11 @@ typedef struct oxav1_gen_config {
     uint8_t exercise_delta_lf; /* random per-mi delta_lf, delta_lf_multi drawn */
     uint8_t cdef_skip_units;   /* some 64x64 units get cdef_idx = -1 */
     uint8_t lr_types;          /* bit0 NONE, bit1 WIENER, bit2 SGRPROJ allowed */
+    uint8_t decoder_geometry;  /* the block and transform geometry a decoder emits:
+                                  HORZ_A/B, VERT_A/B and 4:1 (HORZ_4/VERT_4)
+                                  partitions, transforms up to Max_Tx_Size_Rect
+                                  (the 64-sample family), intra tx_depth splits
+                                  and var-tx split trees in non-skip inter blocks */
 } oxav1_gen_config;
 
 /* One generated frame; all buffers are owned by the object. */

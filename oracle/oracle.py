@@ -21,6 +21,10 @@ from oxav1.dropin import FilterCalls  # noqa: E402
 from oxav1.frames import planes_struct  # noqa: E402
 
 LIB_PATH = os.path.join(HERE, "liboxav1_oracle.so")
+# Built by `make check` with -fsanitize=undefined (no recovery: the first
+# signed overflow, bad shift or out-of-range index aborts the process);
+# selected with OXAV1_ORACLE_SAN=ubsan.
+UBSAN_PATH = os.path.join(HERE, "liboxav1_oracle_ubsan.so")
 
 ORACLE_NAMES = {
     "lf": "oo_loop_filter_frame",
@@ -127,6 +131,22 @@ _SIGS = {
 _lib = None
 
 
+def _bind(lib):
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load_path(path):
+    """A separately built oracle library (a sanitizer or mutant build) with
+    the same signatures; symbols it lacks stay unbound."""
+    return _bind(C.CDLL(path))
+
+
 def build():
     """Compile the oracle with its Makefile (gcc, -O3)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -136,14 +156,20 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    path = LIB_PATH
+    san = os.environ.get("OXAV1_ORACLE_SAN", "")
+    if san:
+        if san != "ubsan":
+            raise RuntimeError(f"OXAV1_ORACLE_SAN={san!r}: only 'ubsan' is built")
+        path = UBSAN_PATH
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", HERE, "liboxav1_oracle_ubsan.so"], check=True)
+    elif not os.path.exists(path):
         build()
-    lib = C.CDLL(LIB_PATH)
-    for name, (res, args) in _SIGS.items():
-        fn = getattr(lib, name)
-        fn.restype = res
-        fn.argtypes = args
-    _lib = lib
+    lib = C.CDLL(path)
+    for name in _SIGS:
+        getattr(lib, name)  # every symbol must be there
+    _lib = _bind(lib)
     return lib
 
 

@@ -155,12 +155,12 @@ static uint8_t lf_strength_selection(const oxav1_lf_ctx *ctx, const oxav1_lf_gri
     if (ctx->loop_filter_delta_enabled) {
         int32_t n_shift = lvl_seg >> 5;
         if (ref_frame == 0) {
-            lvl_seg += (int32_t)ctx->loop_filter_ref_deltas[0] << n_shift;
+            lvl_seg += (int32_t)ctx->loop_filter_ref_deltas[0] * (1 << n_shift);
         } else {
             uint32_t ref_idx = ref_frame < 7 ? ref_frame : 7;
             uint32_t mode_idx = mode_type < 1 ? mode_type : 1;
-            lvl_seg += (int32_t)ctx->loop_filter_ref_deltas[ref_idx] << n_shift;
-            lvl_seg += (int32_t)ctx->loop_filter_mode_deltas[mode_idx] << n_shift;
+            lvl_seg += (int32_t)ctx->loop_filter_ref_deltas[ref_idx] * (1 << n_shift);
+            lvl_seg += (int32_t)ctx->loop_filter_mode_deltas[mode_idx] * (1 << n_shift);
         }
         lvl_seg = clip3(0, 63, lvl_seg);
     }
@@ -1275,7 +1275,7 @@ static void fg_add_noise(const oxav1_film_grain_params *fg, const int32_t *gl, c
                         merged = avg;
                     } else {
                         int32_t comb = avg * ((int32_t)fg->cb_luma_mult - 128) + orig * ((int32_t)fg->cb_mult - 128);
-                        merged = clip3(0, (1 << bd) - 1, (comb >> 6) + (((int32_t)fg->cb_offset - 256) << (bd - 8)));
+                        merged = clip3(0, (1 << bd) - 1, (comb >> 6) + (((int32_t)fg->cb_offset - 256) * (1 << (bd - 8))));
                     }
                     int32_t nz = img[1][(size_t)yy * pwc + xx];
                     nz = round2_i32(oo_fg_scale_lut(lut, 1, merged, bd) * nz, ssh);
@@ -1288,7 +1288,7 @@ static void fg_add_noise(const oxav1_film_grain_params *fg, const int32_t *gl, c
                         merged = avg;
                     } else {
                         int32_t comb = avg * ((int32_t)fg->cr_luma_mult - 128) + orig * ((int32_t)fg->cr_mult - 128);
-                        merged = clip3(0, (1 << bd) - 1, (comb >> 6) + (((int32_t)fg->cr_offset - 256) << (bd - 8)));
+                        merged = clip3(0, (1 << bd) - 1, (comb >> 6) + (((int32_t)fg->cr_offset - 256) * (1 << (bd - 8))));
                     }
                     int32_t nz = img[2][(size_t)yy * pwc + xx];
                     nz = round2_i32(oo_fg_scale_lut(lut, 2, merged, bd) * nz, ssh);

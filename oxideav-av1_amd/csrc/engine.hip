@@ -29,6 +29,7 @@
 #include "av1_tables.h"
 #include "cdef.hpp"
 #include "common.hpp"
+#include "copy_pool.hpp"
 #include "fg.hpp"
 #include "lf.hpp"
 #include "lr.hpp"

@@ -461,7 +461,7 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
     const bool from_luma = fg.chroma_scaling_from_luma != 0;
     const int32_t lm = plane == 1 ? fg.cb_luma_mult : fg.cr_luma_mult;
     const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
-    const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) << (bd - 8);
+    const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) * (1 << (bd - 8));
     const int32_t maxs = (1 << bd) - 1;
     for (int32_t y = yb0, i = ib; y < yb1; y++, i++, src += ip, dst += op) {
         int32_t nz = hval(pc, pl, i);

@@ -53,6 +53,15 @@ int tx_of(uint32_t w, uint32_t h) {
     return -1;
 }
 
+// Split_Tx_Size (cdf.rs:12922-12942) by its closed form: a square
+// transform halves both sides (TX_4X4 stays), a rectangular one halves its
+// longer side.
+int split_tx(int tx) {
+    uint32_t w = oxav1::kTxWidth[tx], h = oxav1::kTxHeight[tx];
+    if (w == h) return w == 4 ? tx : tx_of(w / 2, h / 2);
+    return w > h ? tx_of(w / 2, h) : tx_of(w, h / 2);
+}
+
 int residual_size(int subsize, int plane, int sx, int sy) {
     if (subsize < 0 || subsize >= 22) return -1;
     int subx = plane > 0 ? (sx != 0) : 0, suby = plane > 0 ? (sy != 0) : 0;
@@ -86,22 +95,46 @@ struct Builder {
         : cfg(c), rng(seed), f(fr), mi_rows(fr->desc.mi_rows), mi_cols(fr->desc.mi_cols),
           luma_tx((size_t)fr->desc.mi_rows * fr->desc.mi_cols, 0) {}
 
+    // Stamp transform `tx` over the (h4 x w4) mi footprint at (r, c),
+    // clipped to the frame.
+    void stamp_tx(uint32_t r, uint32_t c, uint32_t w4, uint32_t h4, int tx) {
+        for (uint32_t y = r; y < r + h4 && y < mi_rows; y++)
+            for (uint32_t x = c; x < c + w4 && x < mi_cols; x++)
+                luma_tx[(size_t)y * mi_cols + x] = (uint8_t)tx;
+    }
+
+    // read_var_tx_size (cdf.rs:28720-28790): a split tree of at most
+    // MAX_VARTX_DEPTH = 2 levels below `tx`, each node split with p = 1/2.
+    void var_tx(uint32_t r, uint32_t c, int tx, int depth) {
+        uint32_t w4 = oxav1::kTxWidth[tx] / 4, h4 = oxav1::kTxHeight[tx] / 4;
+        if (tx != 0 && depth < 2 && rng.chance(1, 2)) {
+            int sub = split_tx(tx);
+            uint32_t sw = oxav1::kTxWidth[sub] / 4, sh = oxav1::kTxHeight[sub] / 4;
+            for (uint32_t i = 0; i < h4; i += sh)
+                for (uint32_t j = 0; j < w4; j += sw) var_tx(r + i, c + j, sub, depth + 1);
+        } else {
+            stamp_tx(r, c, w4, h4, tx);
+        }
+    }
+
     void leaf(uint32_t r, uint32_t c, uint32_t w4, uint32_t h4) {
         int bs = block_of(w4, h4);
-        // transform: largest rect (<= 32 per side) or a smaller square
-        uint32_t bw = w4 * 4, bh = h4 * 4;
-        uint32_t mw = bw < 32 ? bw : 32, mh = bh < 32 ? bh : 32;
-        int tx;
-        if (rng.chance(1, 2)) {
-            tx = tx_of(mw, mh);
-        } else {
-            uint32_t m = mw < mh ? mw : mh;
-            uint32_t steps = 0;
-            for (uint32_t t = m; t > 4; t >>= 1) steps++;
-            uint32_t side = m >> rng.range(0, (int32_t)steps);
-            tx = tx_of(side, side);
+        int tx = 0;
+        if (!cfg.decoder_geometry) {
+            // transform: largest rect (<= 32 per side) or a smaller square
+            uint32_t bw = w4 * 4, bh = h4 * 4;
+            uint32_t mw = bw < 32 ? bw : 32, mh = bh < 32 ? bh : 32;
+            if (rng.chance(1, 2)) {
+                tx = tx_of(mw, mh);
+            } else {
+                uint32_t m = mw < mh ? mw : mh;
+                uint32_t steps = 0;
+                for (uint32_t t = m; t > 4; t >>= 1) steps++;
+                uint32_t side = m >> rng.range(0, (int32_t)steps);
+                tx = tx_of(side, side);
+            }
+            if (tx < 0) tx = 0;
         }
-        if (tx < 0) tx = 0;
         bool skip = rng.chance(3, 10);
         bool inter = rng.chance(1, 2);
         uint8_t ref = inter ? (uint8_t)rng.range(1, 7) : 0;
@@ -122,6 +155,82 @@ struct Builder {
                 for (int i = 0; i < 4; i++) f->delta_lf[k * 4 + i] = dl[i];
                 luma_tx[k] = (uint8_t)tx;
             }
+        if (cfg.decoder_geometry) {
+            // Transform sizes as the decoder's syntax yields them: starting
+            // from Max_Tx_Size_Rect (cdf.rs:12870, the 64-sample family
+            // included) in tiles of at most 64x64; intra blocks take a
+            // uniform tx_depth of 0..2 splits (cdf.rs:28440), skipped inter
+            // blocks the maximum, other inter blocks a var-tx split tree.
+            int max_tx = oxav1::kMaxTxSizeRect[bs];
+            uint32_t tw4 = oxav1::kTxWidth[max_tx] / 4, th4 = oxav1::kTxHeight[max_tx] / 4;
+            int uni = max_tx;
+            if (!inter) {
+                int depth = rng.range(0, 2);
+                for (int d = 0; d < depth; d++) uni = split_tx(uni);
+            }
+            for (uint32_t i = 0; i < h4; i += th4)
+                for (uint32_t j = 0; j < w4; j += tw4) {
+                    if (inter && !skip)
+                        var_tx(r + i, c + j, max_tx, 0);
+                    else
+                        stamp_tx(r + i, c + j, tw4, th4, uni);
+                }
+        }
+    }
+
+    // Partitions a decoder meets (decode_partition, §5.11.4): NONE, HORZ,
+    // VERT, SPLIT, HORZ_A/B and VERT_A/B from 16x16 up, HORZ_4/VERT_4 for
+    // 16x16..64x64; a block crossing the frame edge splits, or halves
+    // along the edge it crosses.
+    void node_decoder(uint32_t r, uint32_t c, uint32_t s4) {
+        if (r >= mi_rows || c >= mi_cols) return;
+        if (s4 == 1) {
+            leaf(r, c, 1, 1);
+            return;
+        }
+        const uint32_t h = s4 / 2, q = s4 / 4;
+        const bool has_rows = r + h < mi_rows, has_cols = c + h < mi_cols;
+        auto split = [&] {
+            node_decoder(r, c, h);
+            node_decoder(r, c + h, h);
+            node_decoder(r + h, c, h);
+            node_decoder(r + h, c + h, h);
+        };
+        if (!has_rows || !has_cols) {
+            if (has_cols && rng.chance(1, 2)) {  // HORZ: the bottom half is off-frame
+                leaf(r, c, s4, h);
+            } else if (has_rows && rng.chance(1, 2)) {  // VERT
+                leaf(r, c, h, s4);
+            } else {
+                split();
+            }
+            return;
+        }
+        uint32_t split_p = s4 >= 16 ? 45 : (s4 >= 8 ? 35 : 30);
+        uint32_t pick = rng.next() % 100;
+        if (pick < split_p) {
+            split();
+            return;
+        }
+        const bool ab = s4 >= 4, four = s4 >= 4 && s4 <= 16;
+        int kind = rng.range(0, 8);  // NONE HORZ VERT HA HB VA VB H4 V4
+        if (!ab && kind >= 3) kind = kind % 3;
+        if (!four && kind >= 7) kind = kind - 4;
+        switch (kind) {
+        case 0: leaf(r, c, s4, s4); break;
+        case 1: leaf(r, c, s4, h); leaf(r + h, c, s4, h); break;
+        case 2: leaf(r, c, h, s4); leaf(r, c + h, h, s4); break;
+        case 3: leaf(r, c, h, h); leaf(r, c + h, h, h); leaf(r + h, c, s4, h); break;  // HORZ_A
+        case 4: leaf(r, c, s4, h); leaf(r + h, c, h, h); leaf(r + h, c + h, h, h); break;  // HORZ_B
+        case 5: leaf(r, c, h, h); leaf(r + h, c, h, h); leaf(r, c + h, h, s4); break;  // VERT_A
+        case 6: leaf(r, c, h, s4); leaf(r, c + h, h, h); leaf(r + h, c + h, h, h); break;  // VERT_B
+        case 7:  // HORZ_4
+            for (uint32_t i = 0; i < 4 && r + i * q < mi_rows; i++) leaf(r + i * q, c, s4, q);
+            break;
+        default:  // VERT_4
+            for (uint32_t i = 0; i < 4 && c + i * q < mi_cols; i++) leaf(r, c + i * q, q, s4);
+            break;
+        }
     }
 
     void node(uint32_t r, uint32_t c, uint32_t s4) {
@@ -326,7 +435,12 @@ extern "C" int oxav1_gen_frame_create(const oxav1_gen_config *cfg, uint64_t seed
     Builder b(*cfg, mr.next(), f);
     const uint32_t sb4 = cfg->sb128 ? 32 : 16;
     for (uint32_t r = 0; r < mi_rows; r += sb4)
-        for (uint32_t c = 0; c < mi_cols; c += sb4) b.node(r, c, sb4);
+        for (uint32_t c = 0; c < mi_cols; c += sb4) {
+            if (cfg->decoder_geometry)
+                b.node_decoder(r, c, sb4);
+            else
+                b.node(r, c, sb4);
+        }
     for (size_t k = 0; k < nmi; k++) {
         f->lf_mi[k] = ((uint32_t)f->mi_size[k] << OXAV1_MI_SIZE_SHIFT) |
                       ((uint32_t)(f->skip[k] != 0) << OXAV1_MI_SKIP_BIT) |

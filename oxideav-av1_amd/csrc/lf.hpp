@@ -56,11 +56,11 @@ __device__ __forceinline__ uint32_t lf_level_at(const FrameJob &J, uint32_t row,
     if (J.lf_delta_enabled) {
         const int32_t n_shift = lvl >> 5;
         if (ref == 0) {
-            lvl += (int32_t)J.lf_ref_deltas[0] << n_shift;
+            lvl += (int32_t)J.lf_ref_deltas[0] * (1 << n_shift);
         } else {
             const uint32_t ri = ref < 7 ? ref : 7;
-            lvl += (int32_t)J.lf_ref_deltas[ri] << n_shift;
-            lvl += (int32_t)J.lf_mode_deltas[mode_type] << n_shift;
+            lvl += (int32_t)J.lf_ref_deltas[ri] * (1 << n_shift);
+            lvl += (int32_t)J.lf_mode_deltas[mode_type] * (1 << n_shift);
         }
         lvl = clip3(0, 63, lvl);
     }

@@ -140,7 +140,7 @@ class GenConfig(C.Structure):
         ("enable_lr", u8), ("enable_fg", u8), ("fg_overlap", u8), ("fg_num_y_points", u8),
         ("fg_ar_lag", u8), ("fg_chroma_points", u8), ("noise_amp", u8), ("rect_partitions", u8),
         ("exercise_segments", u8), ("exercise_delta_lf", u8), ("cdef_skip_units", u8),
-        ("lr_types", u8),
+        ("lr_types", u8), ("decoder_geometry", u8),
     ]
 
 

@@ -22,7 +22,7 @@ def gen_config(**kw):
                     lr_uv_shift=0, enable_lf=1, enable_cdef=1, enable_lr=1, enable_fg=0,
                     fg_overlap=0, fg_num_y_points=0, fg_ar_lag=0, fg_chroma_points=0,
                     noise_amp=8, rect_partitions=1, exercise_segments=0, exercise_delta_lf=0,
-                    cdef_skip_units=0, lr_types=7)
+                    cdef_skip_units=0, lr_types=7, decoder_geometry=0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(c, k, v)

@@ -69,6 +69,50 @@ def test_chain_small_mixed(mode, oracle_mod):
     eng.close()
 
 
+GEO_FORMATS = [  # (bit_depth, sx, sy, mono)
+    (8, 1, 1, 0), (10, 1, 1, 0), (12, 1, 1, 0), (8, 1, 0, 0), (10, 0, 0, 0), (12, 1, 0, 0),
+    (8, 1, 1, 1), (10, 0, 0, 0),
+]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_chain_decoder_geometry(mode, oracle_mod):
+    """The decoder's block and transform geometry (4:1 and A/B partitions,
+    transforms up to 64 samples, intra tx_depth, var-tx trees) in all eight
+    formats, 64x64 and 128x128 superblocks: plain, batched and graph
+    replays equal the oracle."""
+    from oxav1.engine import Engine
+    cfgs = []
+    for i, (bd, sx, sy, mono) in enumerate(GEO_FORMATS):
+        cfgs.append((gen_config(upscaled_width=[200, 136, 256, 168][i % 4],
+                                frame_height=[130, 96, 144, 72][i % 4], bit_depth=bd,
+                                subsampling_x=sx, subsampling_y=sy, mono=mono,
+                                noise_amp=8 << (bd - 8), decoder_geometry=1, sb128=i % 2,
+                                superres_denom=8 + (i % 3) * 2, exercise_segments=i % 3 == 1,
+                                enable_fg=i % 2, fg_num_y_points=6, fg_ar_lag=i % 4,
+                                fg_chroma_points=3 * (1 - mono)), 0x6E0 + i))
+    frames = _frames(cfgs)
+    eng = Engine(max_frames=len(frames))
+    if mode == 0:
+        for df in frames:
+            eng.prepare([df])
+            eng.run(0)
+            eng.sync()
+    else:
+        eng.prepare(frames)
+        eng.run(mode)
+        eng.sync()
+        if mode == 2:
+            for _ in range(2):
+                _check(frames, oracle_mod, "decoder geometry, graph replay")
+                for df in frames:
+                    df.clear_outputs()
+                eng.run(2)
+                eng.sync()
+    _check(frames, oracle_mod, f"decoder geometry, mode {mode}")
+    eng.close()
+
+
 @pytest.mark.parametrize("sizes", [[(16, 16)], [(24, 16), (64, 40), (16, 24)], [(40, 24)] * 11])
 def test_chain_batches_smaller_than_the_claim_partitions(sizes, oracle_mod):
     """Batched launches whose stages have fewer work items than there are

@@ -44,12 +44,14 @@ def _fmt_kw(fmt, size):
                 subsampling_y=sy, mono=mono, noise_amp=8 << (bd - 8))
 
 
+@pytest.mark.parametrize("geo", [0, 1], ids=["quadtree", "decoder_geometry"])
 @pytest.mark.parametrize("fi", range(len(FORMATS)))
 @pytest.mark.parametrize("si", range(len(SIZES)))
-def test_loop_filter_parity(fi, si, gpu_calls, oracle_mod):
-    seed = 0x1F00 + 17 * fi + si
+def test_loop_filter_parity(fi, si, geo, gpu_calls, oracle_mod):
+    """decoder_geometry: 4:1 and A/B partitions, 64-sample transforms, var-tx."""
+    seed = 0x1F00 + 17 * fi + si + 0x3000 * geo
     f = _frame(seed, exercise_segments=si % 2, exercise_delta_lf=(si + fi) % 2,
-               **_fmt_kw(FORMATS[fi], SIZES[si]))
+               decoder_geometry=geo, sb128=geo * (si % 2), **_fmt_kw(FORMATS[fi], SIZES[si]))
     got = gpu_calls.loop_filter_frame(f, f.planes)
     want = oracle_mod.calls().loop_filter_frame(f, f.planes)
     assert any(not np.array_equal(w, p) for w, p in zip(want, f.planes)), "oracle filtered nothing"
@@ -196,3 +198,16 @@ def test_frame_driver_through_dropin_calls(case, gpu_calls, oracle_mod):
     ref_o, out_o, _ = oracle_mod.chain(frame)
     _eq(ref_g, ref_o, "pre-grain reference")
     _eq(out_g, out_o, "film-grain output")
+
+
+@pytest.mark.parametrize("case", __import__("lf_edge_cases").all_cases(), ids=lambda c: c.name)
+def test_loop_filter_edge_cases(case, gpu_calls, oracle_mod):
+    """The hand-derived edge-decision cases (onScreen gates, chroma
+    planeSize, applyFilter, filter size from both sides, prev-side level
+    fallback) through the drop-in call: the GPU output is the expected
+    plane exactly."""
+    from oxav1 import abi
+    want = case.expected(oracle_mod.load())
+    st, got = case.run(abi.load().oxav1_pf_loop_filter_frame)
+    abi.check(st, "loop_filter_frame")
+    _eq(got, want, case.name)

@@ -168,21 +168,21 @@ def test_filter_mask_steep_edge_blocks_filter(oo):
 
 
 def test_driver_iterates_correct_edge_count_for_2x2_mi_grid(oo):
-    """Four on-screen edges on a 2x2 mi grid: with is_intra true and a step
-    across the frame, exactly the two vertical and two horizontal interior
-    edge segments are filtered (loop_filter.rs:1183-1233 counts the calls)."""
-    c = _lf_ctx([10, 10, 0, 0], mi=2, num_planes=1)
-    g = _Grids(2, is_intra=1, tx=0, misize=3)  # TX_4X4, BLOCK_8X8
-    a = np.full((8, 8), 100, dtype=np.int32)
-    a[:, 4:] = 104
-    a[4:, :] += 2
-    ps = planes_struct([a])
-    before = a.copy()
-    oo.oo_loop_filter_frame(C.byref(c), C.byref(g.g), ps, 1)
-    changed_cols = set(np.argwhere(a != before)[:, 1])
-    changed_rows = set(np.argwhere(a != before)[:, 0])
-    assert changed_cols <= {2, 3, 4, 5} | set(range(8)) and changed_rows
-    assert (a != before).any()
+    """loop_filter.rs:1183-1233 counts four on-screen edge calls on a 2x2 mi
+    grid.  Restated output-exactly: the plane equals the input with the
+    pinned narrow filter applied along x = 4 on every row and then along
+    y = 4 on every column, and no other sample moves (edges at x = 0 / y = 0
+    are off screen)."""
+    import lf_edge_cases as lc
+    k = lc.case_2x2_grid()
+    want = k.expected(oo)
+    _, got = k.run(oo.oo_loop_filter_frame)
+    assert np.array_equal(got[0], want[0])
+    moved = np.argwhere(want[0] != k.planes[0])
+    assert len(moved) > 0
+    # narrow filter footprint: columns 2..5 (pass 0) or rows 2..5 (pass 1)
+    assert all(2 <= x <= 5 or 2 <= y <= 5 for y, x in moved)
+    assert len(k.edges) == 4 * 4  # four 4-sample edge segments
 
 
 def test_get_tx_size_and_residual_size(oo):

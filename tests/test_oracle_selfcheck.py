@@ -47,14 +47,19 @@ def _lf_both(oracle_mod, f):
     return seq, par
 
 
+@pytest.mark.parametrize("geo", [0, 1], ids=["quadtree", "decoder_geometry"])
 @pytest.mark.parametrize("fi", range(len(FORMATS)))
 @pytest.mark.parametrize("si", range(len(SIZES)))
-def test_deblock_sequential_equals_parallel(fi, si, oracle_mod):
+def test_deblock_sequential_equals_parallel(fi, si, geo, oracle_mod):
+    """decoder_geometry adds 4:1 and A/B partitions, 64-sample transforms and
+    var-tx trees (the adjacent-footprint argument holds for any legal
+    transform tiling: the next edge along a row is a whole transform width
+    away, and that width bounds both edges' filter lengths)."""
     bd, sx, sy, mono = FORMATS[fi]
-    f = _frame(0x5E0 + 31 * fi + si, upscaled_width=SIZES[si][0], frame_height=SIZES[si][1],
-               bit_depth=bd, subsampling_x=sx, subsampling_y=sy, mono=mono,
-               noise_amp=8 << (bd - 8), exercise_segments=si % 2, exercise_delta_lf=fi % 2,
-               sb128=si % 2)
+    f = _frame(0x5E0 + 31 * fi + si + 7777 * geo, upscaled_width=SIZES[si][0],
+               frame_height=SIZES[si][1], bit_depth=bd, subsampling_x=sx, subsampling_y=sy,
+               mono=mono, noise_amp=8 << (bd - 8), exercise_segments=si % 2,
+               exercise_delta_lf=fi % 2, sb128=si % 2, decoder_geometry=geo)
     seq, par, rev = _lf_walks(oracle_mod, f)
     assert any(not np.array_equal(a, b) for a, b in zip(seq, f.planes)), "nothing was filtered"
     for p, (a, b, c) in enumerate(zip(seq, par, rev)):
@@ -96,3 +101,33 @@ def test_frame_driver_mirror_equals_oracle_chain(oracle_mod):
         ref_d, out_d = dropin.frame_driver_chain(calls, f)
         for a, b in zip(ref_o + out_o, ref_d + out_d):
             assert np.array_equal(a, b), kw
+
+
+CHAIN_MATRIX = [  # (bit_depth, sx, sy, mono, superres_denom, fg)
+    (8, 1, 1, 0, 8, 1), (10, 1, 1, 0, 12, 1), (12, 1, 1, 0, 16, 1), (8, 1, 0, 0, 9, 0),
+    (10, 1, 0, 0, 8, 1), (12, 0, 0, 0, 13, 1), (8, 0, 0, 0, 8, 1), (10, 1, 1, 1, 11, 1),
+    (12, 1, 1, 1, 8, 0),
+]
+
+
+@pytest.mark.parametrize("k", range(len(CHAIN_MATRIX)))
+def test_chain_outputs_in_range(k, oracle_mod):
+    """Every stage writes Clip1-ed values (the reason u8/u16 device storage
+    is lossless): the chain's pre-grain and grain outputs stay within
+    [0, 2^bd - 1] over depths, subsamplings, superres and film grain, with
+    the decoder's geometry and the strongest noise.  Under the UBSan build
+    of the oracle (tests/test_oracle_ubsan.py) this also runs every stage's
+    arithmetic through the overflow and shift checks."""
+    bd, sx, sy, mono, den, fg = CHAIN_MATRIX[k]
+    f = _frame(0xC4A1 + k, upscaled_width=[136, 200, 96][k % 3], frame_height=[72, 104, 64][k % 3],
+               bit_depth=bd, subsampling_x=sx, subsampling_y=sy, mono=mono, superres_denom=den,
+               noise_amp=min(255, 64 << (bd - 8)), decoder_geometry=k % 2, sb128=k % 3 == 2,
+               exercise_segments=1, exercise_delta_lf=1, cdef_skip_units=1, enable_fg=fg,
+               fg_num_y_points=14, fg_ar_lag=3, fg_chroma_points=10 * (1 - mono),
+               fg_overlap=k % 2, lr_unit_shift=k % 3, lr_uv_shift=k % 2)
+    ref, out, _ = oracle_mod.chain(f)
+    hi = (1 << bd) - 1
+    for p in range(f.num_planes):
+        assert ref[p].shape == out[p].shape == f.out_shape(p)
+        for a in (ref[p], out[p]):
+            assert a.min() >= 0 and a.max() <= hi, (p, a.min(), a.max())
