@@ -48,6 +48,9 @@ extern "C" {
 #define OXAV1_ERR_UNSUPPORTED 15
 #define OXAV1_ERR_NO_DEVICE 16
 #define OXAV1_ERR_BUSY 17
+/* the frame's intermediates exceed the allocation-free entry point's
+ * scratch (OXAV1_NOALLOC_SCRATCH_BYTES) */
+#define OXAV1_ERR_SCRATCH_TOO_SMALL 18
 
 /* ---------------------------------------------------------------------- */
 /* Shared types                                                            */
@@ -451,6 +454,26 @@ int oxav1_pf_engine_get_tuning(oxav1_pf_engine *e, oxav1_tuning *t, uint32_t *nu
 /* OXAV1_OK when everything enqueued on the engine's stream has finished,
  * OXAV1_ERR_BUSY while work is in flight; never blocks. */
 int oxav1_pf_engine_query(oxav1_pf_engine *e);
+/* Allocation-free variant of oxav1_pf_engine_run_frame, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Runs the plain chain (one launch per stage) over one frame on
+ * hip_stream (NULL = the default stream), asynchronously, with the same
+ * kernels and outputs as oxav1_pf_engine_run_frame.  The frame's
+ * intermediates (deblocked, CDEF and upscaled planes, film-grain scratch)
+ * and its job record live in a fixed device array of
+ * OXAV1_NOALLOC_SCRATCH_BYTES per GPU, held from the library's device
+ * set-up until the process ends; a frame whose intermediates do not fit
+ * returns OXAV1_ERR_SCRATCH_TOO_SMALL before anything is enqueued (a
+ * 3840x2160 10-bit 4:2:0 frame coded 2560 wide with superres and every
+ * stage on needs 79.3 MiB).  Nothing is allocated and nothing blocks: the
+ * call only enqueues kernels (the scratch is written by kernels, not by
+ * memset or copy nodes).  All calls on one GPU share its array and leave
+ * the last call's contents in it: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be
+ * serialised by the caller. */
+#define OXAV1_NOALLOC_SCRATCH_BYTES (80u << 20)
+int oxav1_pf_run_frame_noalloc(const oxav1_frame_desc *desc, const oxav1_frame_dev *dev, void *hip_stream);
 /* Size n frame slots for frames of desc's shape with every stage on, so
  * that prepare with frames no larger never allocates; *bytes (optional)
  * receives the engine's slot bytes. */

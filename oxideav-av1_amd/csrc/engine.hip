@@ -403,6 +403,16 @@ static int hip_fail(hipError_t e, const char *what) {
 static std::mutex g_init_mu;
 static bool g_init_done[64];
 
+// Scratch memory of the allocation-free entry point
+// (oxav1_pf_run_frame_noalloc): one frame's intermediates and job record.
+// Every device the library runs on holds its own copy until the process
+// ends.  Host code never uses the array's name as an address: ensure_device_init
+// takes each device's address with hipGetSymbolAddress and keeps it in
+// g_scratch_ptr[device].
+constexpr size_t kNoAllocScratchBytes = OXAV1_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+static void *g_scratch_ptr[64];
+
 // Constant tables per device, once.
 static int ensure_device_init() {
     int dev = 0;
@@ -435,7 +445,12 @@ static int ensure_device_init() {
             m[b][i] = (uint16_t)r;
         }
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_lfsr_jump), m, sizeof(m)));
-    if (dev < 64) g_init_done[dev] = true;
+    if (dev < 64) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+        g_init_done[dev] = true;
+    }
     return OXAV1_OK;
 }
 
@@ -563,9 +578,34 @@ constexpr uint8_t kClaimStatic = 255;  // items_per_claim value: no atomics, gri
 
 static const char *kStageNames[6] = {"lf", "cdef", "sr", "lr", "fg", "fg_prep"};
 
-static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_frame_dev &v, FrameSlot &slot,
-                     FrameJob &J) {
+// Bump allocation of a frame's intermediates inside a fixed region (the
+// allocation-free entry point's scratch); 256-byte aligned pieces.
+struct ScratchCarve {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+    int take(size_t n, void **out) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || n > cap - off) return OXAV1_ERR_SCRATCH_TOO_SMALL;
+        *out = base + off;
+        used = off + n;
+        return OXAV1_OK;
+    }
+};
+
+// One frame's job record.  Intermediates come from the frame slot (grown on
+// demand) or, when carve is given, from the carve's region; then the
+// film-grain scratch header is returned in *fg_header for the caller to
+// store on the device instead of being copied here.
+static int build_job(const oxav1_tuning &tun, const oxav1_frame_desc &d, const oxav1_frame_dev &v, FrameSlot &slot,
+                     FrameJob &J, ScratchCarve *carve = nullptr, FgScratch *fg_header = nullptr) {
     std::memset(&J, 0, sizeof J);
+    auto take = [&](DevBuf &b, size_t n, void **out) -> int {
+        if (carve) return carve->take(n, out);
+        int st = b.ensure(n);
+        if (st) return st;
+        *out = b.p;
+        return OXAV1_OK;
+    };
     if (d.bit_depth != 8 && d.bit_depth != 10 && d.bit_depth != 12) return OXAV1_ERR_INVALID_ARG;
     if (d.num_planes != 1 && d.num_planes != 3) return OXAV1_ERR_INVALID_ARG;
     if (d.mi_rows == 0 || d.mi_cols == 0 || d.frame_width == 0 || d.frame_height == 0) return OXAV1_ERR_INVALID_ARG;
@@ -632,18 +672,16 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         J.src_pitch[p] = v.src_pitch[p] ? v.src_pitch[p] : J.pad_cols[p];
         const size_t padn = (size_t)J.pad_rows[p] * J.pad_cols[p] * B;
         if (J.lf_plane_on[p]) {
-            int st = slot.deb[p].ensure(padn);
+            int st = take(slot.deb[p], padn, &J.deb[p]);
             if (st) return st;
-            J.deb[p] = slot.deb[p].p;
             J.deb_pitch[p] = J.pad_cols[p];
         } else {
             J.deb[p] = const_cast<void *>(v.src[p]);
             J.deb_pitch[p] = J.src_pitch[p];
         }
         if (J.apply_cdef) {
-            int st = slot.cdf[p].ensure(padn);
+            int st = take(slot.cdf[p], padn, &J.cdf[p]);
             if (st) return st;
-            J.cdf[p] = slot.cdf[p].p;
             J.cdf_pitch[p] = J.pad_cols[p];
         } else {
             J.cdf[p] = J.deb[p];
@@ -652,7 +690,7 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
     }
     // ---- superres ----
     {
-        const uint32_t r = e->tuning.sr_rows_per_item[J.kind == kU8 ? 0 : 1];
+        const uint32_t r = tun.sr_rows_per_item[J.kind == kU8 ? 0 : 1];
         J.sr_rows_per_item = r ? r : 8;
     }
     if (J.use_sr) {
@@ -663,14 +701,12 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
             sr_consts(dw, uw, d.mi_cols, psx, &J.sr_step[p], &J.sr_init[p], &J.sr_maxx[p]);
             J.sr_in_w[p] = dw;
             const size_t outn = (size_t)J.out_rows[p] * J.out_cols[p] * B;
-            int st = slot.upc[p].ensure(outn);
+            int st = take(slot.upc[p], outn, &J.up_cdef[p]);
             if (st) return st;
-            J.up_cdef[p] = slot.upc[p].p;
             J.up_cdef_pitch[p] = J.out_cols[p];
             if (J.uses_lr) {
-                st = slot.upd[p].ensure(outn);
+                st = take(slot.upd[p], outn, &J.up_deb[p]);
                 if (st) return st;
-                J.up_deb[p] = slot.upd[p].p;
                 J.up_deb_pitch[p] = J.out_cols[p];
             }
             J.lr_rows[p] = J.lrc_rows[p] = J.out_rows[p];
@@ -704,7 +740,7 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         J.lr_tile_w[p] = 64;
         if (J.lr_type[p] != OXAV1_RESTORE_NONE) {
             const uint32_t us = d.lr.loop_restoration_size[p];
-            int st = lr_tile_width(us, psy, &J.lr_tile_w[p], e->tuning.lr_tile_w[J.kind == kU8 ? 0 : 1]);
+            int st = lr_tile_width(us, psy, &J.lr_tile_w[p], tun.lr_tile_w[J.kind == kU8 ? 0 : 1]);
             if (st) return st;
             J.lr_unit_size[p] = us;
             J.lr_unit_rows[p] = oxav1_pf_count_units_in_frame(us, rnd2(J.fh, psy));
@@ -722,18 +758,23 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
     J.fg = d.fg;
     const bool fg_stage = J.apply_fg || (J.out[0] != J.ref_out[0]);
     if (J.apply_fg) {
-        int st = slot.fgs.ensure(sizeof(FgScratch));
+        void *fgs = nullptr, *offs = nullptr;
+        int st = take(slot.fgs, sizeof(FgScratch), &fgs);
         if (st) return st;
         const uint32_t nst = ceil_div(ceil_div(J.fh, 2), 16), nbl = ceil_div(ceil_div(J.uw, 2), 16);
-        st = slot.offs.ensure((size_t)nst * nbl);
+        st = take(slot.offs, (size_t)nst * nbl, &offs);
         if (st) return st;
-        J.fgs = (FgScratch *)slot.fgs.p;
+        J.fgs = (FgScratch *)fgs;
         FgScratch hs;
         std::memset(&hs, 0, sizeof hs);
         hs.nstripes = nst;
         hs.nblocks = nbl;
-        hs.offsets = (uint8_t *)slot.offs.p;
-        HIP_TRY(hipMemcpy(slot.fgs.p, &hs, sizeof hs, hipMemcpyHostToDevice));
+        hs.offsets = (uint8_t *)offs;
+        if (carve) {
+            if (fg_header) *fg_header = hs;
+        } else {
+            HIP_TRY(hipMemcpy(fgs, &hs, sizeof hs, hipMemcpyHostToDevice));
+        }
     }
     // ---- item tables ----
     uint32_t acc = 0;
@@ -775,7 +816,6 @@ static int build_job(oxav1_pf_engine *e, const oxav1_frame_desc &d, const oxav1_
         if (p < np && fg_stage) acc += ceil_div(J.out_rows[p], 64) * J.fg_tiles_x[p];
     }
     J.item_base[kStageFg][3] = acc;
-    (void)e;
     return OXAV1_OK;
 }
 
@@ -878,7 +918,7 @@ extern "C" int oxav1_pf_engine_prepare(oxav1_pf_engine *e, const oxav1_frame_des
     e->jobs_h.resize(n);
     e->nsub = 0;
     for (uint32_t f = 0; f < n; f++) {
-        int st = build_job(e, desc[f], dev[f], e->slots[f], e->jobs_h[f]);
+        int st = build_job(e->tuning, desc[f], dev[f], e->slots[f], e->jobs_h[f]);
         if (st) return st;
     }
     int st = load_batch(e);
@@ -1251,6 +1291,7 @@ extern "C" const char *oxav1_pf_strerror(int status) {
     case OXAV1_ERR_UNSUPPORTED: return "unsupported parameter combination";
     case OXAV1_ERR_NO_DEVICE: return "no HIP device";
     case OXAV1_ERR_BUSY: return "busy: work still in flight";
+    case OXAV1_ERR_SCRATCH_TOO_SMALL: return "frame too large for the allocation-free entry point's scratch";
     default: return "unknown status";
     }
 }
@@ -1383,3 +1424,4 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
 
 #include "dropin.inc"
 #include "elect.inc"
+#include "noalloc.inc"

@@ -23,6 +23,7 @@ ERR_ADMISSION_REFUSED = 14
 ERR_UNSUPPORTED = 15
 ERR_NO_DEVICE = 16
 ERR_BUSY = 17
+ERR_SCRATCH_TOO_SMALL = 18
 
 RESTORE_NONE, RESTORE_WIENER, RESTORE_SGRPROJ, RESTORE_SWITCHABLE = 0, 1, 2, 3
 MI_SIZE_SHIFT, MI_SKIP_BIT, MI_INTRA_BIT, MI_REF_SHIFT, MI_MODE_SHIFT, MI_SEG_SHIFT = 0, 5, 6, 8, 12, 24
@@ -254,6 +255,7 @@ _SIGS = {
     "oxav1_pf_engine_prepare": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev), u32]),
     "oxav1_pf_engine_run": (C.c_int, [C.c_void_p, C.c_int]),
     "oxav1_pf_engine_run_frame": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev)]),
+    "oxav1_pf_run_frame_noalloc": (C.c_int, [_P(FrameDesc), _P(FrameDev), C.c_void_p]),
     "oxav1_pf_engine_run_batch": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev), u32, C.c_int]),
     "oxav1_pf_engine_sync": (C.c_int, [C.c_void_p]),
     "oxav1_pf_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
