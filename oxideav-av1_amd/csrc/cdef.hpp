@@ -16,6 +16,11 @@ namespace oxav1 {
 constexpr int kCdefUnit = 64;           // luma samples per unit side
 constexpr int kCdefB = 2;               // tap reach
 constexpr int kCdefLw = kCdefUnit + 2 * kCdefB;  // 68
+// The engine's u16 LDS tile is stored twice, the second copy kCdefDual
+// (odd) elements after the first, so a column pair starting at an odd
+// element of the first copy starts at an even one of the second: every pair
+// the packed filter reads is an aligned dword.
+constexpr int kCdefDual = kCdefLw * kCdefLw + 1;
 
 __constant__ int8_t c_cdef_dir[8][2][2] = {
     {{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}}, {{0, 1}, {1, 2}},
@@ -240,6 +245,129 @@ __device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int
     }
 }
 
+// Two samples per lane: the interior path (every tap inside the plane) of
+// the engine's u8/u16 frames, with the LDS tile in u16.  Lanes own column
+// pairs (xx, xx + 1), xx even, which always lie in one block (blocks are at
+// least 4 samples wide), so both samples share the block's taps, strengths
+// and damping shifts.  Every intermediate fits a signed 16-bit half: tap
+// differences |d| < 2^12, thresholds <= 240 (15 << 4 at 12-bit), and the
+// weighted sum |sum| <= 12 * 240 + 12 * 64 = 3648; the damping shift is
+// capped at 15, which like the scalar cap of 31 leaves 0 for any |d| <
+// 2^15.  The pair of a tap at an odd offset is an unaligned dword of LDS.
+// Per tap, constrain (cdef.rs:592-606) is d clamped to [-t, t] with t =
+// max(0, thr - (|d| >> adj)) (an unsigned saturating subtract).
+typedef short cdef_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short cdef_u16x2 __attribute__((ext_vector_type(2)));
+
+// A block's taps for the packed path: byte offsets of its twelve taps (+
+// and - of each CdefTaps::off) from a pair's own dword in the first copy
+// (into the first copy for even element offsets, into the second for odd
+// ones), the strengths and damping shifts (capped at 15) and the primary
+// weights.
+struct CdefTapsPk {
+    int16_t boff[12];
+    uint16_t pri, sec;
+    uint8_t pri_adj, sec_adj, pt0, pt1;
+    uint8_t active, pad[3];
+};
+
+__device__ __forceinline__ CdefTapsPk cdef_taps_pk(const CdefTaps &t) {
+    CdefTapsPk k{};
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int sg = 0; sg < 2; sg++) {
+            const int32_t o = sg ? -t.off[i] : t.off[i];
+            k.boff[2 * i + sg] = (int16_t)(2 * o + ((o & 1) ? 2 * kCdefDual : 0));
+        }
+    k.pri = (uint16_t)t.pri;
+    k.sec = (uint16_t)t.sec;
+    k.pri_adj = (uint8_t)min((int32_t)t.pri_adj, 15);
+    k.sec_adj = (uint8_t)min((int32_t)t.sec_adj, 15);
+    k.pt0 = t.pt0;
+    k.pt1 = t.pt1;
+    k.active = t.active;
+    return k;
+}
+
+// The unit's per-block tables share one LDS region: the direction costs
+// (64 x 8 words) until the blocks' decisions are made, then either the
+// scalar or the packed taps of the plane being filtered.
+union CdefTables {
+    int32_t costs[64 * 8];
+    CdefTaps taps[64];
+    CdefTapsPk pk[64];
+};
+
+template <typename T>
+__device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, int tw, int th, int32_t ox, int32_t oy,
+                                                     int32_t bw_log2, int32_t bh_log2, const CdefTapsPk *taps,
+                                                     int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
+    using s16x2 = cdef_s16x2;
+    using u16x2 = cdef_u16x2;
+    const int pr_log2 = row_log2 - 1;  // column pairs per tile row (log2)
+    const int xx = 2 * ((int)threadIdx.x & ((1 << pr_log2) - 1));
+    const int ystep = blockDim.x >> pr_log2;
+    if (xx >= tw) return;
+    const int y_first = (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) << (6 - pr_log2)) +
+                        ((int)(threadIdx.x & 63) >> pr_log2);
+    auto pair_at = [](const uint16_t *q, int32_t byte_off) -> s16x2 {
+        return __builtin_bit_cast(s16x2, *(const uint32_t *)((const char *)q + byte_off));
+    };
+    auto splat = [](int32_t v) -> uint32_t { return (uint32_t)v * 0x10001u; };
+    for (int yy = y_first; yy < th; yy += ystep) {
+        const CdefTapsPk b = taps[(yy >> bh_log2) * 8 + (xx >> bw_log2)];
+        const uint16_t *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
+        const s16x2 x = pair_at(c, 0);
+        s16x2 res = x;
+        if (b.active) {
+            const u16x2 thr_p = __builtin_bit_cast(u16x2, splat(b.pri));
+            const u16x2 thr_s = __builtin_bit_cast(u16x2, splat(b.sec));
+            const u16x2 adj_p = __builtin_bit_cast(u16x2, splat(b.pri_adj));
+            const u16x2 adj_s = __builtin_bit_cast(u16x2, splat(b.sec_adj));
+            s16x2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};  // primary k = 0, primary k = 1, secondary
+            s16x2 sec1 = {0, 0};                      // secondary k = 1 (weight 1; k = 0 weighs 2)
+            s16x2 mx = x, mn = x;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const u16x2 thr = i < 2 ? thr_p : thr_s;
+                const u16x2 adj = i < 2 ? adj_p : adj_s;
+#pragma unroll
+                for (int sg = 0; sg < 2; sg++) {
+                    const s16x2 v = pair_at(c, b.boff[2 * i + sg]);
+                    const s16x2 d = v - x;
+                    const s16x2 ad = __builtin_elementwise_max(d, x - v);
+                    const u16x2 tt = __builtin_elementwise_sub_sat(thr, __builtin_bit_cast(u16x2, ad) >> adj);
+                    const s16x2 ts = __builtin_bit_cast(s16x2, tt);
+                    const s16x2 cv = __builtin_elementwise_max(__builtin_elementwise_min(d, ts), -ts);
+                    if (i < 2)
+                        acc[i] += cv;
+                    else if (i < 4)
+                        acc[2] += cv;
+                    else
+                        sec1 += cv;
+                    mx = __builtin_elementwise_max(mx, v);
+                    mn = __builtin_elementwise_min(mn, v);
+                }
+            }
+            const s16x2 w0 = __builtin_bit_cast(s16x2, splat(b.pt0));
+            const s16x2 w1 = __builtin_bit_cast(s16x2, splat(b.pt1));
+            const s16x2 sum = acc[0] * w0 + acc[1] * w1 + (acc[2] + acc[2]) + sec1;
+            // (8 + sum - (sum < 0)) >> 4, arithmetic shifts on each half
+            const s16x2 neg = sum >> (s16x2){15, 15};
+            const s16x2 delta = (sum + neg + (s16x2){8, 8}) >> (s16x2){4, 4};
+            res = __builtin_elementwise_min(__builtin_elementwise_max(x + delta, mn), mx);
+        }
+        const int32_t py = oy + yy, px = ox + xx;
+        const uint32_t r = __builtin_bit_cast(uint32_t, res);
+        if constexpr (std::is_same<T, uint16_t>::value) {
+            *(G<uint32_t> *)(dst + (size_t)py * dpitch + px) = r;  // px and the pitch are even
+        } else {
+            *(G<uint16_t> *)(dst + (size_t)py * dpitch + px) = (uint16_t)((r & 0xffu) | ((r >> 8) & 0xff00u));
+        }
+    }
+}
+
 template <typename L, typename T, bool CHECK>
 __device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, int th, int32_t ox, int32_t oy, int32_t pw,
                                   int32_t ph, int32_t bw_log2, int32_t bh_log2, const CdefTaps *taps,
@@ -253,7 +381,7 @@ __device__ __forceinline__ int div_magic(int i, uint32_t m) {
     return (int)(((uint32_t)i * m) >> 20);
 }
 
-template <typename L, typename T>
+template <typename L, typename T, bool DUAL = false>
 __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
                           const G<T> *__restrict__ src, uint32_t spitch) {
     const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
@@ -265,14 +393,22 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
         const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
         // Out-of-plane entries are never used as taps; clamp keeps the load in bounds.
         const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
-        t[i] = (L)src[(size_t)yc * spitch + xc];
+        const L v = (L)src[(size_t)yc * spitch + xc];
+        t[i] = v;
+        if (DUAL) t[kCdefDual + i] = v;
     }
 }
+
+// The engine's u8/u16 frames filter interior units two samples per lane.
+template <typename T>
+constexpr bool kCdefPacked = std::is_same<T, uint8_t>::value || std::is_same<T, uint16_t>::value;
 
 // One 64x64 unit (ur, uc in 64-luma units): deb -> cdf for all planes.
 template <typename T>
 __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32_t uc, typename LdsOf<T>::type *lds,
-                          CdefBlk *blk, CdefTaps *taps, int32_t *costs /* 64 x 8 */) {
+                          CdefBlk *blk, CdefTables *tabs) {
+    int32_t *costs = tabs->costs;
+    CdefTaps *taps = tabs->taps;
     using L = typename LdsOf<T>::type;
     const int tid = threadIdx.x;
     const uint32_t r0 = ur * 16, c0 = uc * 16;  // mi coordinates
@@ -291,7 +427,7 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
     const int lw0 = min(kCdefUnit, pw0 - ox0), lh0 = min(kCdefUnit, ph0 - oy0);
     const int ls0 = lw0 + 2 * kCdefB;
     L *t0 = lds;
-    cdef_load<L, T>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
+    cdef_load<L, T, kCdefPacked<T>>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
     __syncthreads();
     // per 8x8 block: skip test (cdef.rs:266-272, out-of-grid cells count as
     // skip) and the direction search, split over the four waves (two
@@ -342,15 +478,28 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
     // luma filter
     {
         const int32_t damping = (int32_t)J.cdef.cdef_damping + coeff_shift;
-        if (tid < 64) taps[tid] = cdef_taps(blk[tid], false, damping, coeff_shift, ls0);
-        __syncthreads();
         const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
+        const bool packed = kCdefPacked<T> && inside && !(tw0 & 1) && !(ls0 & 1) && !(J.cdf_pitch[0] & 1);
+        if (tid < 64) {
+            const CdefTaps tp = cdef_taps(blk[tid], false, damping, coeff_shift, ls0);
+            if (packed)
+                tabs->pk[tid] = cdef_taps_pk(tp);
+            else
+                taps[tid] = tp;
+        }
+        __syncthreads();
         G<T> *dst = plane_ptr<T>(J.cdf[0]);
+        if constexpr (kCdefPacked<T>)
+            if (packed) {
+                cdef_filter_plane_pk<T>(t0, ls0, tw0, th0, ox0, oy0, 3, 3, tabs->pk, 6, dst, J.cdf_pitch[0]);
+                goto chroma;
+            }
         if (inside)
             cdef_filter_plane<L, T, false>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, taps, 6, dst, J.cdf_pitch[0]);
         else
             cdef_filter_plane<L, T, true>(t0, ls0, tw0, th0, ox0, oy0, pw0, ph0, 3, 3, taps, 6, dst, J.cdf_pitch[0]);
     }
+chroma:
     if (J.np < 2) return;
     // chroma planes (cdef.rs:296-313), each staged in turn
     const int32_t sx = J.sx, sy = J.sy;
@@ -363,12 +512,27 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
         const int lw = min(kCdefUnit >> sx, pw - ox), lh = min(kCdefUnit >> sy, ph - oy);
         __syncthreads();
         if (tw <= 0 || th <= 0) continue;
-        cdef_load<L, T>(t0, lw, lh, ox, oy, pw, ph, plane_ptr<T>(J.deb[p]), J.deb_pitch[p]);
+        cdef_load<L, T, kCdefPacked<T>>(t0, lw, lh, ox, oy, pw, ph, plane_ptr<T>(J.deb[p]), J.deb_pitch[p]);
         const int ls = lw + 2 * kCdefB;
-        if (p == 1 && tid < 64) taps[tid] = cdef_taps(blk[tid], true, damping_uv, coeff_shift, ls);
-        __syncthreads();
+        // the Cb and Cr tiles have the same shape, so Cr reuses Cb's taps
         const bool inside = ox >= kCdefB && oy >= kCdefB && ox + tw + kCdefB <= pw && oy + th + kCdefB <= ph;
+        const bool packed = kCdefPacked<T> && inside && !(tw & 1) && !(ls & 1) && !(J.cdf_pitch[1] & 1) &&
+                            !(J.cdf_pitch[2] & 1);
+        if (p == 1 && tid < 64) {
+            const CdefTaps tp = cdef_taps(blk[tid], true, damping_uv, coeff_shift, ls);
+            if (packed)
+                tabs->pk[tid] = cdef_taps_pk(tp);
+            else
+                taps[tid] = tp;
+        }
+        __syncthreads();
         G<T> *dst = plane_ptr<T>(J.cdf[p]);
+        if constexpr (kCdefPacked<T>)
+            if (packed) {
+                cdef_filter_plane_pk<T>(t0, ls, tw, th, ox, oy, 3 - sx, 3 - sy, tabs->pk, 6 - sx, dst,
+                                        J.cdf_pitch[p]);
+                continue;
+            }
         if (inside)
             cdef_filter_plane<L, T, false>(t0, ls, tw, th, ox, oy, pw, ph, 3 - sx, 3 - sy, taps, 6 - sx, dst,
                                            J.cdf_pitch[p]);

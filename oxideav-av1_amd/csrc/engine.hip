@@ -82,9 +82,9 @@ __device__ __forceinline__ void lf_item(const FrameJob &J, uint32_t local, typen
 
 template <typename T>
 __device__ __forceinline__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
-                          CdefTaps *taps, int32_t *costs) {
+                          CdefTables *tabs) {
     const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
-    cdef_unit<T>(J, ur, uc, tile, blk, taps, costs);
+    cdef_unit<T>(J, ur, uc, tile, blk, tabs);
 }
 
 // Superres items: planes 0..2 upscale the CDEF frame in runs of
@@ -190,8 +190,8 @@ __device__ __forceinline__ void claim_items(uint32_t *counter, uint32_t total, u
             if (left_ == 0) {                                                     \
                 if (threadIdx.x == 0) claim_items(counter, total, claim, part_, s_item, s_n); \
                 __syncthreads();                                                  \
-                base_ = s_item;                                                   \
-                left_ = s_n ? s_n : 1u;                                           \
+                base_ = __builtin_amdgcn_readfirstlane(s_item);                   \
+                left_ = __builtin_amdgcn_readfirstlane(s_n ? s_n : 1u);           \
                 __syncthreads();                                                  \
             }                                                                     \
             item = base_++;                                                       \
@@ -236,18 +236,18 @@ template <int KIND>
 __device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter, uint32_t claim) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
-    __shared__ L tile[kCdefLw * kCdefLw];
+    // u16 tiles are stored twice (packed filter path, kCdefDual)
+    __shared__ __align__(16) L tile[KIND == 1 ? kCdefLw * kCdefLw : 2 * kCdefLw * kCdefLw + 2];
     __shared__ CdefBlk blk[64];
-    __shared__ CdefTaps taps[64];
-    __shared__ int32_t costs[64 * 8];
+    __shared__ CdefTables tabs;
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            cdef_item<int32_t>(J, ref.local, tile, blk, taps, costs);
+            cdef_item<int32_t>(J, ref.local, tile, blk, &tabs);
         } else {
             if (J.kind == kU8)
-                cdef_item<uint8_t>(J, ref.local, tile, blk, taps, costs);
+                cdef_item<uint8_t>(J, ref.local, tile, blk, &tabs);
             else
-                cdef_item<uint16_t>(J, ref.local, tile, blk, taps, costs);
+                cdef_item<uint16_t>(J, ref.local, tile, blk, &tabs);
         }
     })
 }
@@ -428,6 +428,11 @@ static int ensure_device_init() {
     uint16_t a2[256] = {0};
     for (int z = 1; z < 255; z++) a2[z] = (uint16_t)(((z << 8) + z / 2) / (z + 1));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_a2), a2, sizeof(a2)));
+    LrSgrConsts sc[16][2];
+    for (int set = 0; set < 16; set++)
+        for (int pass = 0; pass < 2; pass++)
+            sc[set][pass] = lr_sgr_consts_of(kSgrParams[set * 4 + 2 * pass], kSgrParams[set * 4 + 2 * pass + 1]);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_consts), sc, sizeof(sc)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), kGaussianSequence, sizeof(kGaussianSequence)));
     // LFSR step matrix (film_grain.rs:117-130) and its powers 2^b.
     uint16_t m[16][16];

@@ -80,27 +80,50 @@ struct LrTileGeom {
 
 // Self-guided A/B from box sums (loop_restoration.rs:615-665): a = sum of
 // squares, b = sum over the (2r+1)^2 window.  Returns lr_pack_ab(A, B).
+// K24: samples are u8/u16 (the engine's frames), so the bounds below hold
+// and the products run on the 24-bit multiplier; the drop-in path's int32
+// samples keep 32-bit multiplies.
+template <bool K24>
 __device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, int32_t n, uint32_t s, uint32_t one_over_n,
                                           uint32_t bd, const uint16_t *a2tab) {
+    auto mul = [](int32_t x, int32_t y) -> int32_t { return K24 ? lr_mul24(x, y) : x * y; };
     const uint32_t sha = 2 * (bd - 8), shb = bd - 8;
     const uint32_t ar = sha ? ((a + (1u << (sha - 1))) >> sha) : a;
     const int32_t dr = shb ? ((b + (1 << (shb - 1))) >> shb) : b;
-    int32_t p = (int32_t)ar * n - dr * dr;
+    // ar < 2^21 (at most 25 squares of 12-bit samples, shifted by 8), n <= 25
+    // and |dr| <= 25 * 255 + 1 at every depth: 24-bit multiplies
+    int32_t p = mul((int32_t)ar, n) - mul(dr, dr);
     p = p < 0 ? 0 : p;
     // p * s < 2^32 at every depth (p < 41.2e6 with s <= 76 for r = 2, p < 5.3e6
     // with s <= 15 for r = 1)
     const uint32_t z = ((uint32_t)p * s + (1u << 19)) >> 20;
     const uint32_t a2 = z >= 255 ? 256u : (z == 0 ? 1u : (uint32_t)a2tab[z]);
-    // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth
-    const uint32_t b2 = (256u - a2) * (uint32_t)b * one_over_n;
+    // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth;
+    // (256 - a2) * oneOverN <= 255 * 455 and b < 2^17, so both factors run on
+    // the 24-bit multiplier
+    const uint32_t b2 = (uint32_t)mul(mul((int32_t)(256u - a2), (int32_t)one_over_n), b);
     return lr_pack_ab(a2, (b2 + 2048u) >> 12);
 }
 
-__device__ __forceinline__ void lr_sgr_consts(int32_t r, int32_t e, int32_t *n, uint32_t *s, uint32_t *oon) {
-    *n = (2 * r + 1) * (2 * r + 1);
-    const int64_t n2e = (int64_t)(*n) * (*n) * e;
-    *s = (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
-    *oon = (uint32_t)(((1 << 12) + *n / 2) / *n);
+// n, s = Round2-style (2^20 + n^2 e / 2) / (n^2 e) and oneOverN =
+// (2^12 + n / 2) / n of one self-guided pass (loop_restoration.rs:620-640),
+// per (set, pass): they depend on Sgr_Params only, so the host computes
+// them once at device set-up instead of every thread per work item (a 64-bit
+// division each).
+struct LrSgrConsts {
+    int32_t n;
+    uint32_t s, oon;
+};
+__constant__ LrSgrConsts c_sgr_consts[16][2];
+
+__host__ __device__ inline LrSgrConsts lr_sgr_consts_of(int32_t r, int32_t e) {
+    LrSgrConsts c{};
+    if (r == 0) return c;
+    c.n = (2 * r + 1) * (2 * r + 1);
+    const int64_t n2e = (int64_t)c.n * c.n * e;
+    c.s = (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
+    c.oon = (uint32_t)(((1 << 12) + c.n / 2) / c.n);
+    return c;
 }
 
 // One tile: (plane, stripe, column tile) -> out (cropped frame coordinates
@@ -256,8 +279,11 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
             const int wr = rb + wv;
             if (wr < H + 6 && lane < W) {
                 const L *q = S.src + wr * kLrSw + lane;
-                const int32_t acc = hf[0] * ((int32_t)q[0] + (int32_t)q[6]) + hf[1] * ((int32_t)q[1] + (int32_t)q[5]) +
-                                    hf[2] * ((int32_t)q[2] + (int32_t)q[4]) + hf[3] * (int32_t)q[3];
+                // u16 samples: |taps| < 2^9 and sample sums < 2^17, 24-bit multiplies
+                constexpr bool k24h = std::is_same<L, uint16_t>::value;
+                auto hm = [](int32_t x, int32_t y) -> int32_t { return k24h ? lr_mul24(x, y) : x * y; };
+                const int32_t acc = hm(hf[0], (int32_t)q[0] + (int32_t)q[6]) + hm(hf[1], (int32_t)q[1] + (int32_t)q[5]) +
+                                    hm(hf[2], (int32_t)q[2] + (int32_t)q[4]) + hm(hf[3], (int32_t)q[3]);
                 S.inter[wr * kLrMaxW + lane] = clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
             }
         }
@@ -302,10 +328,9 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
         const int KR = (H + 2 + nb - 1) / nb;
         const int k0 = -1 + band * KR, k1 = min(H, k0 + KR - 1);  // inclusive tile rows
         if (band < nb && k0 <= k1) {
-            int32_t n0 = 0, n1 = 0;
-            uint32_t s0 = 0, s1 = 0, o0 = 0, o1 = 0;
-            if (r0) lr_sgr_consts(r0, e0, &n0, &s0, &o0);
-            if (r1) lr_sgr_consts(r1, e1, &n1, &s1, &o1);
+            const LrSgrConsts kc0 = c_sgr_consts[set][0], kc1 = c_sgr_consts[set][1];
+            const int32_t n0 = kc0.n, n1 = kc1.n;
+            const uint32_t s0 = kc0.s, s1 = kc1.s, o0 = kc0.oon, o1 = kc1.oon;
             // window row of tile row k is k + 3; column of tile col c - 1 is c + 2
             int32_t h3[5], q3[5], h5[5], q5[5];
             auto hsum = [&](int wr, int slot) {
@@ -339,13 +364,13 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
                             a = (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
                             b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
                         }
-                        S.ab1[(k + 1) * kLrAw + c] = lr_ab(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
+                        S.ab1[(k + 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value>(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
                     }
                     if (r0 && ((ys + k) & 1)) {
                         const uint32_t a =
                             (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
                         const int32_t b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
-                        S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
+                        S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value>(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
                     }
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
