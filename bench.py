@@ -66,15 +66,20 @@ def host_info():
     return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
 
 
-def dropin_rate(frame):
+def dropin_rate(frame, expect=None):
     """The reference's interface end to end: the frame driver's post-filter
     sequence through the five drop-in calls on host int32 planes (each call
-    stages its planes to HBM and back over PCIe).  Returns a dict."""
+    stages its planes to HBM and back over PCIe).  expect: the engine's
+    (pre-grain, output) planes of the same frame, compared with the calls'
+    result.  Returns a dict."""
     import numpy as np
 
     from oxav1 import dropin
     calls = dropin.gpu()
-    dropin.frame_driver_chain(calls, frame)  # warm
+    ref_d, out_d = dropin.frame_driver_chain(calls, frame)  # warm
+    same = None
+    if expect is not None:
+        same = all(np.array_equal(a, b) for a, b in zip(ref_d + out_d, expect[0] + expect[1]))
     reps, stage = 3, []
     t = time.perf_counter()
     for _ in range(reps):
@@ -86,6 +91,7 @@ def dropin_rate(frame):
     return {"mpx_s": round(d.upscaled_width * d.frame_height / s / 1e6, 2), "ms_per_frame": round(s * 1e3, 2),
             "stage_ms": {k: round(v * 1e3, 2) for k, v in zip(("lf", "cdef", "sr", "lr", "fg"),
                                                               np.mean(stage, axis=0))},
+            "equals_engine_output": same,
             "what": f"one {d.upscaled_width}x{d.frame_height} frame through the five drop-in calls "
                     f"(int32 host planes, PCIe-inclusive), the reference's own interface"}
 
@@ -393,6 +399,8 @@ def main():
     ap.add_argument("--frames", type=int, default=8, help="frames per step (batch)")
     ap.add_argument("--cfg", type=int, default=2, help="workload configuration (2 = 4K 10-bit)")
     ap.add_argument("--cpu-frames", type=int, default=4, help="frames timed on the CPU oracle (about 10 s)")
+    ap.add_argument("--cpu-reps", type=int, default=0,
+                    help="repetitions of the CPU sample (0: 50 for frames under 1 Mpixel, else 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-parallel", type=int, default=1,
                     help="also time the CPU oracle frame-parallel over the host cores (1/0)")
@@ -544,9 +552,11 @@ def main():
         n = min(args.cpu_frames, len(frames))
         # evenly spaced over the batch, so a mixed batch's sample covers each of its blocks
         pick = [i * len(frames) // n for i in range(n)]
+        reps = args.cpu_reps or (50 if max(fpx[i] for i in pick) < 1_000_000 else 1)
         t = time.perf_counter()
-        outs = [oracle.chain(frames[i]) for i in pick]
-        cpu_s = time.perf_counter() - t
+        for _ in range(reps):
+            outs = [oracle.chain(frames[i]) for i in pick]
+        cpu_s = (time.perf_counter() - t) / reps
         oracle_exact = 0
         for i, (ref_o, out_o, _) in zip(pick, outs):
             ref_g, out_g = plain[i]
@@ -556,7 +566,8 @@ def main():
         cpu = {"value": round(sum(fpx[i] for i in pick) / cpu_s / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
                "kind": "port",
                "sample": f"{which} of the {len(frames)} frames of the step ({describe([frames[i] for i in pick])}), "
-                         f"full chain through the C oracle, single thread, {cpu_s:.1f} s",
+                         f"full chain through the C oracle, single thread, "
+                         + (f"{reps} repetitions, {cpu_s * 1e3:.2f} ms each" if reps > 1 else f"{cpu_s:.1f} s"),
                "stage_ms_per_frame": {k: round(v * 1e-6, 1) for k, v in zip(("lf", "cdef", "sr", "lr", "fg"),
                                                                           stage_ns)},
                "host": host_info()}
@@ -573,7 +584,7 @@ def main():
                                      "sample": f"all {len(frames)} frames of the step, one frame per "
                                                f"thread, {par_s:.1f} s"}
 
-    dropin = None if args.no_dropin else dropin_rate(frames[0])
+    dropin = None if args.no_dropin else dropin_rate(frames[0], plain[0])
 
     value = total_px / s_batch / 1e6
     line = {

@@ -277,6 +277,16 @@ int oxav1_pf_film_grain_synthesis(const oxav1_film_grain_params *fg, uint8_t bit
                                   uint8_t subsampling_y, uint8_t matrix_coefficients,
                                   oxav1_plane_i32 *planes, uint32_t num_plane_bufs);
 
+/* Staging of the five calls above: every sample the decoder passes is a
+ * Clip1-ed value in [0, 2^bit_depth), so a call whose int32 input planes
+ * all lie in that range stages them as u8 (8-bit) or u16 planes, converted
+ * on the host while copying, and runs the u8/u16 kernels; the results are
+ * widened back to int32.  A call with any sample out of range stages int32
+ * planes and runs the int32 kernels.  Results are the same either way.
+ * enable = 0 forces the int32 path for every call (default: 1).  No
+ * counterpart in the reference (its planes are always i32). */
+int oxav1_pf_dropin_narrowing(int enable);
+
 /* ---------------------------------------------------------------------- */
 /* Library / device                                                        */
 /* ---------------------------------------------------------------------- */

@@ -1,5 +1,5 @@
-// Host-side parallel memcpy used to stage the drop-in calls' pageable planes
-// through pinned buffers.  Host code only (no HIP); shared by engine.hip and
+// Host-side parallel memcpy (and element-wise conversions) used to stage the
+// drop-in calls' pageable planes through pinned buffers.  Host code only (no HIP); shared by engine.hip and
 // the CPU fork-safety test (tests/test_copy_pool.py).
 #pragma once
 
@@ -16,7 +16,7 @@
 
 namespace oxav1 {
 
-// Each copy is split over up to seven worker threads and the calling thread.
+// Each copy is split over up to fifteen worker threads and the calling thread.
 // A single-threaded pageable copy runs at one core's memcpy rate; a 4K
 // frame's five drop-in calls move about 0.6 GB of int32 samples.
 //
@@ -43,10 +43,23 @@ class CopyPool {
 
     // dst[0, n) = src[0, n)
     void copy(void *dst, const void *src, size_t n) {
+        struct Ctx {
+            char *dst;
+            const char *src;
+        } c{(char *)dst, (const char *)src};
+        run(n, 4096, [](void *p, size_t lo, size_t hi) {
+            const Ctx &c = *(const Ctx *)p;
+            std::memcpy(c.dst + lo, c.src + lo, hi - lo);
+        }, &c);
+    }
+
+    // fn(ctx, lo, hi) over slices of [0, n) whose bounds are multiples of
+    // `align`, on the workers and the calling thread; returns when all are done.
+    void run(size_t n, size_t align, void (*fn)(void *, size_t, size_t), void *ctx) {
         const pid_t me = getpid();
         Shared *s = s_.load(std::memory_order_acquire);
         if (s && s->owner != me) {  // forked child
-            std::memcpy(dst, src, n);
+            if (n) fn(ctx, 0, n);
             return;
         }
         if (!s) {
@@ -56,14 +69,15 @@ class CopyPool {
                 s = new Shared;
                 s->owner = me;
                 const unsigned hc = std::thread::hardware_concurrency();
-                const int nw = (int)std::min(7u, hc > 1 ? hc - 1 : 1u);
+                const int nw = (int)std::min(15u, hc > 1 ? hc - 1 : 1u);
                 for (int i = 0; i < nw; i++) s->th.emplace_back([s, i] { work(s, i + 1); });
                 s_.store(s, std::memory_order_release);
             }
         }
         std::unique_lock<std::mutex> l(s->m);
-        s->dst = (char *)dst;
-        s->src = (const char *)src;
+        s->fn = fn;
+        s->ctx = ctx;
+        s->align = align ? align : 1;
         s->n = n;
         s->parts = (int)s->th.size() + 1;
         s->pending = (int)s->th.size();
@@ -84,15 +98,16 @@ class CopyPool {
         uint64_t gen = 0;
         int pending = 0, parts = 1;
         bool stop = false;
-        char *dst = nullptr;
-        const char *src = nullptr;
-        size_t n = 0;
+        void (*fn)(void *, size_t, size_t) = nullptr;
+        void *ctx = nullptr;
+        size_t align = 1, n = 0;
     };
     static void part(Shared *s, int i) {
-        // 4 KiB-aligned slices so the threads touch separate pages
-        const size_t per = ((s->n + s->parts - 1) / s->parts + 4095) & ~(size_t)4095;
+        // aligned slices (4 KiB for byte copies) so the threads touch separate pages
+        const size_t a = s->align;
+        const size_t per = ((s->n + s->parts - 1) / s->parts + a - 1) / a * a;
         const size_t lo = std::min(s->n, per * i), hi = std::min(s->n, lo + per);
-        if (hi > lo) std::memcpy(s->dst + lo, s->src + lo, hi - lo);
+        if (hi > lo) s->fn(s->ctx, lo, hi);
     }
     static void work(Shared *s, int i) {
         uint64_t seen = 0;

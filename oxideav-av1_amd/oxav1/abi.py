@@ -256,6 +256,7 @@ _SIGS = {
     "oxav1_pf_engine_run": (C.c_int, [C.c_void_p, C.c_int]),
     "oxav1_pf_engine_run_frame": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev)]),
     "oxav1_pf_run_frame_noalloc": (C.c_int, [_P(FrameDesc), _P(FrameDev), C.c_void_p]),
+    "oxav1_pf_dropin_narrowing": (C.c_int, [C.c_int]),
     "oxav1_pf_engine_run_batch": (C.c_int, [C.c_void_p, _P(FrameDesc), _P(FrameDev), u32, C.c_int]),
     "oxav1_pf_engine_sync": (C.c_int, [C.c_void_p]),
     "oxav1_pf_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),

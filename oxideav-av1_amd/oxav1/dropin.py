@@ -33,8 +33,13 @@ class FilterCalls:
         return ret
 
     # loop_filter_frame (loop_filter.rs:271), in place -> returns new arrays
-    def loop_filter_frame(self, frame, planes, ctx=None):
-        out = [np.ascontiguousarray(p, dtype=np.int32).copy() for p in planes]
+    # (inplace=True filters the caller's int32 planes themselves)
+    def loop_filter_frame(self, frame, planes, ctx=None, inplace=False):
+        if inplace:
+            assert all(p.dtype == np.int32 and p.flags.c_contiguous for p in planes)
+            out = list(planes)
+        else:
+            out = [np.ascontiguousarray(p, dtype=np.int32).copy() for p in planes]
         c, g, keep = frame.lf_args()
         if ctx is not None:
             c = ctx
@@ -111,7 +116,7 @@ def frame_driver_chain(calls, frame, stage_s=None):
     t = time.perf_counter()
     rec = [p.copy() for p in frame.planes]
     if d.apply_lf:
-        rec = calls.loop_filter_frame(frame, rec)
+        rec = calls.loop_filter_frame(frame, rec, inplace=True)
     times.append(time.perf_counter() - t)
     t = time.perf_counter()
     uses_lr = bool(d.lr.uses_lr)

@@ -15,6 +15,17 @@ from conftest import gen_config
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(params=[1, 0], ids=["narrowed", "int32"], autouse=True)
+def staging(request):
+    """Every test runs twice: with the calls' narrowed staging (u8/u16
+    planes and kernels, the default) and with the int32 path forced."""
+    from oxav1 import abi
+    lib = abi.load()
+    lib.oxav1_pf_dropin_narrowing(request.param)
+    yield request.param
+    lib.oxav1_pf_dropin_narrowing(1)
+
 FORMATS = [  # (bit_depth, sx, sy, mono)
     (8, 1, 1, 0), (10, 1, 1, 0), (12, 1, 1, 0), (8, 1, 0, 0), (10, 0, 0, 0), (12, 1, 0, 0),
     (8, 1, 1, 1), (10, 0, 0, 0),
@@ -211,3 +222,46 @@ def test_loop_filter_edge_cases(case, gpu_calls, oracle_mod):
     st, got = case.run(abi.load().oxav1_pf_loop_filter_frame)
     abi.check(st, "loop_filter_frame")
     _eq(got, want, case.name)
+
+
+@pytest.mark.parametrize("stage", ["lf", "cdef", "sr", "lr", "fg"])
+def test_out_of_range_samples_take_the_int32_path(stage, gpu_calls, oracle_mod):
+    """A sample outside [0, 2^bd) (here 2^bd + 4 and -3) makes a call stage
+    int32 planes and run the int32 kernels: the output still equals the
+    oracle's on the same int32 input."""
+    f = _frame(0x0F0 + len(stage), upscaled_width=160, frame_height=72, bit_depth=10, noise_amp=32,
+               superres_denom=11, enable_fg=1, fg_num_y_points=8, fg_ar_lag=2, fg_chroma_points=4)
+    hi = (1 << 10) + 4
+    oc = oracle_mod.calls()
+    if stage in ("lf", "cdef"):
+        pl = [p.copy() for p in f.planes]
+        pl[0][9, 13] = hi
+        pl[1][4, 5] = -3
+        fn = "loop_filter_frame" if stage == "lf" else "cdef_frame"
+        _eq(getattr(gpu_calls, fn)(f, pl), getattr(oc, fn)(f, pl), stage)
+    elif stage == "sr":
+        pl = [p.copy() for p in f.planes]
+        pl[0][2, 20] = hi
+        shapes = [f.out_shape(p) for p in range(f.num_planes)]
+        st_g, out_g = gpu_calls.upscale_frame(f.sr_ctx(), pl, shapes)
+        st_o, out_o = oc.upscale_frame(f.sr_ctx(), pl, shapes)
+        assert st_g == st_o == 0
+        _eq(out_g, out_o, "sr")
+    elif stage == "lr":
+        shapes = [f.out_shape(p) for p in range(f.num_planes)]
+        rng = np.random.default_rng(3)
+        cur = [rng.integers(0, 1 << 10, size=sh).astype(np.int32) for sh in shapes]
+        cdf = [rng.integers(0, 1 << 10, size=sh).astype(np.int32) for sh in shapes]
+        cdf[0][40, 40] = hi
+        cur[2][10, 11] = -3
+        _eq(gpu_calls.loop_restoration_frame(f, cur, cdf, lr_init=cdf),
+            oc.loop_restoration_frame(f, cur, cdf, lr_init=cdf), "lr")
+    else:
+        d = f.desc
+        shapes = [f.out_shape(p) for p in range(f.num_planes)]
+        rng = np.random.default_rng(4)
+        pl = [rng.integers(0, 1 << 10, size=sh).astype(np.int32) for sh in shapes]
+        pl[0][7, 8] = hi
+        pl[1][3, 3] = -3
+        args = (d.fg, d.bit_depth, d.num_planes, d.subsampling_x, d.subsampling_y, d.matrix_coefficients)
+        _eq(gpu_calls.film_grain_synthesis(*args, pl), oc.film_grain_synthesis(*args, pl), "fg")
