@@ -310,6 +310,10 @@ size_t oxav1_pf_struct_size(const char *name);
  * allocator of their own (bindings, tests, benchmarks).  Synchronous. */
 int oxav1_pf_dev_alloc(void **ptr, size_t bytes);
 int oxav1_pf_dev_free(void *ptr);
+/* Page-locked host memory (hipHostMalloc), for the host buffers of
+ * oxav1_pf_server_submit_host: copies from and to it run asynchronously. */
+int oxav1_pf_host_alloc(void **ptr, size_t bytes);
+int oxav1_pf_host_free(void *ptr);
 int oxav1_pf_dev_upload(void *dst, const void *src, size_t bytes);
 int oxav1_pf_dev_download(void *dst, const void *src, size_t bytes);
 int oxav1_pf_dev_fill(void *dst, int byte_value, size_t bytes);
@@ -579,6 +583,42 @@ int oxav1_pf_server_submit(oxav1_pf_server *s, uint32_t session, const oxav1_fra
  * Without grain (fg->apply_grain == 0) the stored frame is copied. */
 int oxav1_pf_server_submit_existing(oxav1_pf_server *s, uint32_t session, uint32_t slot,
                                     const oxav1_film_grain_params *fg, double arrival_us, uint64_t *ticket);
+/* One frame as the CPU stages of the decoder leave it, in host memory
+ * (page-locked, oxav1_pf_host_alloc, so the copies are asynchronous): the
+ * pre-filter planes (u8 at 8-bit, u16 above; the padded CurrFrame,
+ * (mi_rows*4)>>sy x (mi_cols*4)>>sx, pitch in samples, 0 = tight) and the
+ * side information laid out as in oxav1_frame_dev.  ref_out / out are
+ * optional host destinations (cropped output shape, tight rows) of the
+ * pre-grain frame and the film-grain output, filled before the frame is
+ * reported done (frame_driver.rs:1098-1116 hands the narrowed output back to
+ * the caller); NULL leaves the result only in the session's slot. */
+typedef struct oxav1_frame_host {
+    const void *src[3];
+    uint32_t src_pitch[3];
+    const uint32_t *lf_mi;
+    const int8_t *delta_lf;      /* NULL = no delta_lf */
+    const uint8_t *lf_tx_size[3];
+    const int8_t *cdef_idx;
+    const oxav1_lr_unit *lr_units[3];
+    void *ref_out[3];
+    void *out[3];
+} oxav1_frame_host;
+
+/* Queue one frame whose inputs are in host memory: the server copies them
+ * into one of the session's input buffers (its input region, opened with
+ * input_bytes, holds input_bytes / oxav1_pf_server_frame_input_bytes such
+ * buffers) on its upload stream at once, so the copy overlaps the frames
+ * already on the device; the lane that takes the frame waits for that copy,
+ * and copies the results back to the host destinations after the chain.
+ * OXAV1_ERR_BUSY when the slot is busy or every input buffer holds a frame
+ * still queued or in flight. */
+int oxav1_pf_server_submit_host(oxav1_pf_server *s, uint32_t session, const oxav1_frame_desc *desc,
+                                const oxav1_frame_host *in, uint32_t slot, double arrival_us, uint64_t *ticket);
+/* Device bytes one frame of this shape needs in a session's input region
+ * for oxav1_pf_server_submit_host (planes and side information, 256-byte
+ * aligned pieces). */
+uint64_t oxav1_pf_server_frame_input_bytes(const oxav1_frame_desc *shape);
+
 /* Retire finished lanes, launch ready frames on free lanes, and return up
  * to cap finished frames (the rest are kept for the next poll). */
 int oxav1_pf_server_poll(oxav1_pf_server *s, oxav1_server_done *done, uint32_t cap, uint32_t *n_done);

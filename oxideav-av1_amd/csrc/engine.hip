@@ -1342,6 +1342,18 @@ extern "C" int oxav1_pf_dev_alloc(void **ptr, size_t bytes) {
     HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
     return OXAV1_OK;
 }
+extern "C" int oxav1_pf_host_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return OXAV1_ERR_INVALID_ARG;
+    *ptr = nullptr;
+    int st = ensure_device_init();
+    if (st) return st;
+    HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return OXAV1_OK;
+}
+extern "C" int oxav1_pf_host_free(void *ptr) {
+    if (ptr) HIP_TRY(hipHostFree(ptr));
+    return OXAV1_OK;
+}
 extern "C" int oxav1_pf_dev_free(void *ptr) {
     if (ptr) HIP_TRY(hipFree(ptr));
     return OXAV1_OK;
@@ -1414,6 +1426,7 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_film_grain_params") return sizeof(oxav1_film_grain_params);
     if (n == "oxav1_frame_desc") return sizeof(oxav1_frame_desc);
     if (n == "oxav1_frame_dev") return sizeof(oxav1_frame_dev);
+    if (n == "oxav1_frame_host") return sizeof(oxav1_frame_host);
     if (n == "oxav1_server_config") return sizeof(oxav1_server_config);
     if (n == "oxav1_tuning") return sizeof(oxav1_tuning);
     if (n == "oxav1_walker_grids") return sizeof(oxav1_walker_grids);

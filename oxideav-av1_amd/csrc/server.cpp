@@ -69,6 +69,63 @@ struct Arena {
     }
 };
 
+uint32_t rnd2(uint32_t x, uint32_t n);
+
+// Where each piece of one frame's inputs sits inside an input buffer
+// (oxav1_pf_server_submit_host): the three planes, then the side
+// information, 256-byte aligned.
+struct InputLayout {
+    uint64_t src[3], src_bytes[3], src_rows[3], src_cols[3];
+    uint64_t lf_mi, nmi, delta_lf, tx[3], tx_bytes[3], cdef_idx, cdef_bytes, lr[3], lr_bytes[3];
+    uint64_t total = 0;
+};
+
+// lr_max: room for the most restoration units any frame of this shape can
+// signal (every plane restored with 32-sample units), so one layout serves
+// all the session's frames.
+InputLayout input_layout(const oxav1_frame_desc &d, bool lr_max) {
+    InputLayout L{};
+    uint64_t at = 0;
+    auto take = [&](uint64_t n) {
+        const uint64_t o = at;
+        at += align_up(n ? n : 1);
+        return o;
+    };
+    const uint64_t B = d.bit_depth == 8 ? 1 : 2;
+    const uint32_t sx = d.subsampling_x ? 1 : 0, sy = d.subsampling_y ? 1 : 0;
+    for (uint32_t p = 0; p < 3; p++) {
+        const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+        L.src_rows[p] = p < d.num_planes ? (d.mi_rows * 4) >> psy : 0;
+        L.src_cols[p] = p < d.num_planes ? (d.mi_cols * 4) >> psx : 0;
+        L.src_bytes[p] = L.src_rows[p] * L.src_cols[p] * B;
+        L.src[p] = take(L.src_bytes[p]);
+    }
+    L.nmi = (uint64_t)d.mi_rows * d.mi_cols;
+    L.lf_mi = take(L.nmi * 4);
+    L.delta_lf = take(L.nmi * 4);
+    for (uint32_t p = 0; p < 3; p++) {
+        const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+        L.tx_bytes[p] = (uint64_t)((d.mi_rows + psy) >> psy) * ((d.mi_cols + psx) >> psx);
+        L.tx[p] = take(L.tx_bytes[p]);
+    }
+    L.cdef_bytes = (uint64_t)((d.mi_rows + 15) >> 4) * ((d.mi_cols + 15) >> 4);
+    L.cdef_idx = take(L.cdef_bytes);
+    for (uint32_t p = 0; p < 3; p++) {
+        L.lr_bytes[p] = 0;
+        if (p < d.num_planes &&
+            (lr_max || (d.lr.uses_lr && d.lr.frame_restoration_type[p] != OXAV1_RESTORE_NONE))) {
+            const uint32_t psx = p ? sx : 0, psy = p ? sy : 0;
+            const uint32_t us = lr_max ? 32u : d.lr.loop_restoration_size[p];
+            const uint32_t ur = oxav1_pf_count_units_in_frame(us, rnd2(d.frame_height, psy));
+            const uint32_t uc = oxav1_pf_count_units_in_frame(us, rnd2(std::max(d.upscaled_width, d.frame_width), psx));
+            L.lr_bytes[p] = (uint64_t)ur * uc * sizeof(oxav1_lr_unit);
+        }
+        L.lr[p] = take(L.lr_bytes[p]);
+    }
+    L.total = at;
+    return L;
+}
+
 struct Session {
     bool open = false;
     oxav1_frame_desc shape;
@@ -79,6 +136,11 @@ struct Session {
     std::vector<char *> ref, out;  // nslots * 3 plane pointers
     std::vector<uint8_t> busy;     // per slot: queued or in flight
     uint32_t pending = 0;
+    // input region: carved into host-submission input buffers on first use
+    char *input = nullptr;
+    uint64_t input_bytes = 0;
+    std::vector<uint8_t> in_busy;
+    std::vector<hipEvent_t> in_ev;  // each buffer's upload
 };
 
 struct Pending {
@@ -87,6 +149,8 @@ struct Pending {
     oxav1_frame_desc desc;
     oxav1_frame_dev dev;
     double arrival;
+    int inbuf = -1;                   // host submission: the input buffer it occupies
+    void *host_ref[3] = {nullptr, nullptr, nullptr}, *host_out[3] = {nullptr, nullptr, nullptr};
 };
 
 struct Lane {
@@ -115,6 +179,7 @@ bool same_shape(const oxav1_frame_desc &a, const oxav1_frame_desc &b) {
 
 struct oxav1_pf_server {
     int device = 0;
+    hipStream_t upload = nullptr;  // host submissions' input copies
     uint32_t max_batch = 16;
     Arena arena;
     void *arena_mem = nullptr;
@@ -138,6 +203,11 @@ extern "C" int oxav1_pf_server_destroy(oxav1_pf_server *s) {
             (void)oxav1_pf_engine_sync(l.eng);
             (void)oxav1_pf_engine_destroy(l.eng);
         }
+    if (s->upload) (void)hipStreamSynchronize(s->upload);
+    for (auto &ss : s->sessions)
+        for (auto ev : ss.in_ev)
+            if (ev) (void)hipEventDestroy(ev);
+    if (s->upload) (void)hipStreamDestroy(s->upload);
     if (s->arena_mem) (void)oxav1_pf_dev_free(s->arena_mem);
     delete s;
     return OXAV1_OK;
@@ -188,6 +258,10 @@ extern "C" int oxav1_pf_server_create(oxav1_pf_server **out, const oxav1_server_
         return st;
     }
     s->arena.reset((char *)s->arena_mem, want);
+    if (hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess) {
+        oxav1_pf_server_destroy(s);
+        return OXAV1_ERR_HIP;
+    }
     *out = s;
     return OXAV1_OK;
 }
@@ -229,6 +303,8 @@ extern "C" int oxav1_pf_server_open_session(oxav1_pf_server *s, const oxav1_fram
                 at += plane_bytes[p];
             }
     if (input) *input = input_bytes ? at : nullptr;
+    ss.input = input_bytes ? at : nullptr;
+    ss.input_bytes = input_bytes;
     if (footprint) *footprint = ss.bytes;
     uint32_t id = 0;
     while (id < s->sessions.size() && s->sessions[id].open) id++;
@@ -243,6 +319,8 @@ extern "C" int oxav1_pf_server_close_session(oxav1_pf_server *s, uint32_t sessio
     Session &ss = s->sessions[session];
     if (ss.pending) return OXAV1_ERR_BUSY;
     s->arena.release(ss.off, ss.bytes);
+    for (auto ev : ss.in_ev)
+        if (ev) (void)hipEventDestroy(ev);
     ss = Session();
     return OXAV1_OK;
 }
@@ -329,6 +407,101 @@ extern "C" int oxav1_pf_server_submit_existing(oxav1_pf_server *s, uint32_t sess
     return OXAV1_OK;
 }
 
+extern "C" uint64_t oxav1_pf_server_frame_input_bytes(const oxav1_frame_desc *shape) {
+    return shape ? input_layout(*shape, true).total : 0;
+}
+
+extern "C" int oxav1_pf_server_submit_host(oxav1_pf_server *s, uint32_t session, const oxav1_frame_desc *desc,
+                                           const oxav1_frame_host *in, uint32_t slot, double arrival_us,
+                                           uint64_t *ticket) {
+    if (!s || !desc || !in || session >= s->sessions.size() || !s->sessions[session].open)
+        return OXAV1_ERR_INVALID_ARG;
+    Session &ss = s->sessions[session];
+    if (slot >= ss.nslots || !same_shape(*desc, ss.shape)) return OXAV1_ERR_INVALID_ARG;
+    // the session's fixed layout (offsets, stride) and this frame's sizes
+    const InputLayout L = input_layout(ss.shape, true), F = input_layout(*desc, false);
+    if (!ss.input || L.total > ss.input_bytes || F.nmi != L.nmi) return OXAV1_ERR_INVALID_ARG;
+    for (uint32_t p = 0; p < 3; p++)
+        if (F.lr_bytes[p] > L.lr_bytes[p] || F.src_bytes[p] != L.src_bytes[p]) return OXAV1_ERR_INVALID_ARG;
+    if (!in->lf_mi || !in->cdef_idx || !in->lf_tx_size[0]) return OXAV1_ERR_INVALID_ARG;
+    for (uint32_t p = 0; p < desc->num_planes; p++)
+        if (!in->src[p] || (p && !in->lf_tx_size[p]) || (F.lr_bytes[p] && !in->lr_units[p]))
+            return OXAV1_ERR_INVALID_ARG;
+    if (ss.busy[slot]) return OXAV1_ERR_BUSY;
+    if (ss.in_busy.empty()) {  // carve the input region on first use
+        const uint64_t n = ss.input_bytes / L.total;
+        ss.in_busy.assign(n, 0);
+        ss.in_ev.assign(n, nullptr);
+        for (auto &ev : ss.in_ev)
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return OXAV1_ERR_HIP;
+    }
+    int b = -1;
+    for (size_t i = 0; i < ss.in_busy.size(); i++)
+        if (!ss.in_busy[i]) {
+            b = (int)i;
+            break;
+        }
+    if (b < 0) return OXAV1_ERR_BUSY;
+    char *base = ss.input + (uint64_t)b * L.total;
+    const uint64_t B = desc->bit_depth == 8 ? 1 : 2;
+    hipStream_t u = s->upload;
+    Pending f;
+    std::memset(&f.dev, 0, sizeof f.dev);
+    for (uint32_t p = 0; p < desc->num_planes; p++) {
+        char *dst = base + L.src[p];
+        const uint64_t row = L.src_cols[p] * B, pitch = in->src_pitch[p] ? in->src_pitch[p] * B : row;
+        if (pitch == row) {
+            if (hipMemcpyAsync(dst, in->src[p], L.src_bytes[p], hipMemcpyHostToDevice, u) != hipSuccess)
+                return OXAV1_ERR_HIP;
+        } else if (hipMemcpy2DAsync(dst, row, in->src[p], pitch, row, L.src_rows[p], hipMemcpyHostToDevice, u) !=
+                   hipSuccess) {
+            return OXAV1_ERR_HIP;
+        }
+        f.dev.src[p] = dst;
+        f.dev.src_pitch[p] = (uint32_t)L.src_cols[p];
+    }
+    auto put = [&](uint64_t off, const void *src, uint64_t n) -> bool {
+        return !n || hipMemcpyAsync(base + off, src, n, hipMemcpyHostToDevice, u) == hipSuccess;
+    };
+    bool ok = put(L.lf_mi, in->lf_mi, L.nmi * 4) && put(L.cdef_idx, in->cdef_idx, L.cdef_bytes);
+    if (in->delta_lf) ok = ok && put(L.delta_lf, in->delta_lf, L.nmi * 4);
+    for (uint32_t p = 0; p < 3 && ok; p++) {
+        const bool on = p < desc->num_planes;
+        if (on) ok = put(L.tx[p], in->lf_tx_size[p], L.tx_bytes[p]);
+        if (F.lr_bytes[p]) ok = ok && put(L.lr[p], in->lr_units[p], F.lr_bytes[p]);
+    }
+    if (!ok || hipEventRecord(ss.in_ev[b], u) != hipSuccess) return OXAV1_ERR_HIP;
+    f.dev.lf_mi = (const uint32_t *)(base + L.lf_mi);
+    f.dev.delta_lf = in->delta_lf ? (const int8_t *)(base + L.delta_lf) : nullptr;
+    for (uint32_t p = 0; p < 3; p++) {
+        f.dev.lf_tx_size[p] = p < desc->num_planes ? (const uint8_t *)(base + L.tx[p]) : nullptr;
+        f.dev.lr_units[p] = F.lr_bytes[p] ? (const oxav1_lr_unit *)(base + L.lr[p]) : nullptr;
+    }
+    f.dev.cdef_idx = (const int8_t *)(base + L.cdef_idx);
+    f.ticket = s->next_ticket++;
+    f.session = session;
+    f.slot = slot;
+    f.desc = *desc;
+    for (uint32_t p = 0; p < 3; p++) {
+        const bool on = p < ss.shape.num_planes;
+        f.dev.ref_out[p] = on ? ss.ref[slot * 3 + p] : nullptr;
+        f.dev.ref_pitch[p] = on ? ss.cols[p] : 0;
+        f.dev.out[p] = on && ss.grain ? ss.out[slot * 3 + p] : nullptr;
+        f.dev.out_pitch[p] = on && ss.grain ? ss.cols[p] : 0;
+        f.host_ref[p] = on ? in->ref_out[p] : nullptr;
+        f.host_out[p] = on && ss.grain ? in->out[p] : nullptr;
+    }
+    f.arrival = arrival_us;
+    f.inbuf = b;
+    ss.in_busy[b] = 1;
+    ss.busy[slot] = 1;
+    ss.pending++;
+    s->ready.push_back(f);
+    s->submitted++;
+    if (ticket) *ticket = f.ticket;
+    return OXAV1_OK;
+}
+
 static void retire(oxav1_pf_server *s, Lane &l, int status, double t) {
     const uint32_t n = (uint32_t)l.frames.size();
     for (const Pending &f : l.frames) {
@@ -344,6 +517,7 @@ static void retire(oxav1_pf_server *s, Lane &l, int status, double t) {
         s->finished.push_back(r);
         Session &ss = s->sessions[f.session];
         ss.busy[f.slot] = 0;
+        if (f.inbuf >= 0) ss.in_busy[f.inbuf] = 0;
         ss.pending--;
         s->done++;
     }
@@ -378,7 +552,27 @@ extern "C" int oxav1_pf_server_poll(oxav1_pf_server *s, oxav1_server_done *done,
         }
         l.launch = now_us();
         int st = oxav1_pf_engine_prepare(l.eng, s->dbuf.data(), s->vbuf.data(), n);
+        hipStream_t es = (hipStream_t)oxav1_pf_engine_stream(l.eng);
+        // host submissions: the chain waits for the frames' input copies
+        for (const Pending &f : l.frames)
+            if (!st && f.inbuf >= 0 &&
+                hipStreamWaitEvent(es, s->sessions[f.session].in_ev[f.inbuf], 0) != hipSuccess)
+                st = OXAV1_ERR_HIP;
         if (!st) st = oxav1_pf_engine_run(l.eng, 1);
+        // results copied back to the host destinations before the lane drains
+        for (const Pending &f : l.frames) {
+            const Session &ss = s->sessions[f.session];
+            const uint64_t B = ss.shape.bit_depth == 8 ? 1 : 2;
+            for (uint32_t p = 0; p < ss.shape.num_planes && !st; p++) {
+                const uint64_t n = (uint64_t)ss.rows[p] * ss.cols[p] * B;
+                if (f.host_ref[p] &&
+                    hipMemcpyAsync(f.host_ref[p], f.dev.ref_out[p], n, hipMemcpyDeviceToHost, es) != hipSuccess)
+                    st = OXAV1_ERR_HIP;
+                if (f.host_out[p] && f.dev.out[p] &&
+                    hipMemcpyAsync(f.host_out[p], f.dev.out[p], n, hipMemcpyDeviceToHost, es) != hipSuccess)
+                    st = OXAV1_ERR_HIP;
+            }
+        }
         if (st) {
             retire(s, l, st, now_us());
             if (!first_err) first_err = st;

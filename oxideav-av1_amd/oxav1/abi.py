@@ -133,6 +133,14 @@ class FrameDev(C.Structure):
     ]
 
 
+class FrameHost(C.Structure):
+    _fields_ = [
+        ("src", C.c_void_p * 3), ("src_pitch", u32 * 3), ("lf_mi", C.c_void_p), ("delta_lf", C.c_void_p),
+        ("lf_tx_size", C.c_void_p * 3), ("cdef_idx", C.c_void_p), ("lr_units", C.c_void_p * 3),
+        ("ref_out", C.c_void_p * 3), ("out", C.c_void_p * 3),
+    ]
+
+
 class GenConfig(C.Structure):
     _fields_ = [
         ("upscaled_width", u32), ("frame_height", u32), ("bit_depth", u8), ("subsampling_x", u8),
@@ -228,7 +236,7 @@ ABI_STRUCTS = {
     "oxav1_cdef_params": CdefParams, "oxav1_cdef_ctx": CdefCtx, "oxav1_cdef_grids": CdefGrids,
     "oxav1_sr_ctx": SrCtx, "oxav1_lr_unit": LrUnit, "oxav1_lr_ctx": LrCtx,
     "oxav1_lr_grids": LrGrids, "oxav1_film_grain_params": FilmGrainParams,
-    "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev,
+    "oxav1_frame_desc": FrameDesc, "oxav1_frame_dev": FrameDev, "oxav1_frame_host": FrameHost,
     "oxav1_tuning": Tuning, "oxav1_walker_grids": WalkerGrids, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
     "oxav1_server_stats": ServerStats, "oxav1_cdef_cand": CdefCand, "oxav1_cdef_ssd_frame": CdefSsdFrame,
     "oxav1_cdef_elect_input": CdefElectInput, "oxav1_cdef_election": CdefElection,
@@ -277,6 +285,11 @@ _SIGS = {
     "oxav1_pf_server_slot": (C.c_int, [C.c_void_p, u32, u32, u32, C.c_int, _P(C.c_void_p), _P(u32)]),
     "oxav1_pf_server_submit": (
         C.c_int, [C.c_void_p, u32, _P(FrameDesc), _P(FrameDev), u32, C.c_double, _P(u64)]),
+    "oxav1_pf_server_submit_host": (
+        C.c_int, [C.c_void_p, u32, _P(FrameDesc), _P(FrameHost), u32, C.c_double, _P(u64)]),
+    "oxav1_pf_server_frame_input_bytes": (u64, [_P(FrameDesc)]),
+    "oxav1_pf_host_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
+    "oxav1_pf_host_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_server_submit_existing": (
         C.c_int, [C.c_void_p, u32, u32, _P(FilmGrainParams), C.c_double, _P(u64)]),
     "oxav1_pf_server_poll": (C.c_int, [C.c_void_p, _P(ServerDone), u32, _P(u32)]),

@@ -4,8 +4,13 @@ take what is ready when they free up).
 
 A DecodeSession stands in for one decoder: the CPU produces its pre-filter
 frames (seeded mode info and parameters, a pool of distinct frames reused
-in order), which sit in the session's input region of the server's arena,
-and its frames arrive as a Poisson process.  Frame k of a session uses pool
+in order) and its frames arrive as a Poisson process.  With host=True the
+frames live in page-locked host memory, as the decoder's CPU stages leave
+them: every submission copies its frame to one of the session's input
+buffers on the server's upload stream while other frames are on the device,
+and the results are copied back to host memory before the frame is done
+(frame_driver.rs:790-824, 1098-1116).  Otherwise the pool sits in the
+session's input region of the server's arena, uploaded once.  Frame k of a session uses pool
 frame k % pool and writes reference slot k % slots; the slots stay resident
 in HBM for the session's lifetime (the pre-grain reference store of
 frame_driver.rs:1044-1052).
@@ -76,6 +81,63 @@ def upload_inputs(frame, base):
     return v
 
 
+class HostArray:
+    """A numpy array over page-locked host memory (oxav1_pf_host_alloc), so
+    the server's copies from and to it are asynchronous."""
+
+    def __init__(self, shape, dtype):
+        self.lib = abi.load()
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        abi.check(self.lib.oxav1_pf_host_alloc(C.byref(p), max(n, 1)), "host_alloc")
+        self.ptr = p.value
+        self.a = np.frombuffer((C.c_char * max(n, 1)).from_address(self.ptr), dtype=dt,
+                               count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.a = None
+            self.lib.oxav1_pf_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostFrame:
+    """One pre-filter frame as the CPU stages of a decoder leave it: planes
+    (u8/u16) and side information in page-locked host memory."""
+
+    def __init__(self, frame):
+        d = frame.desc
+        st = np.uint8 if d.bit_depth == 8 else np.uint16
+        self.bufs = []
+
+        def put(a, dtype):
+            h = HostArray(a.shape, dtype)
+            h.a[...] = a
+            self.bufs.append(h)
+            return h.ptr
+
+        v = abi.FrameHost()
+        for p in range(d.num_planes):
+            v.src[p] = put(frame.planes[p].astype(st), st)
+            v.src_pitch[p] = frame.planes[p].shape[1]
+        v.lf_mi = put(frame.lf_mi, np.uint32)
+        v.delta_lf = put(frame.delta_lf, np.int8)
+        for p in range(3):
+            v.lf_tx_size[p] = put(frame.lf_tx_size[p], np.uint8)
+            u = frame.lr_units[p]
+            if u is not None:
+                v.lr_units[p] = put(np.frombuffer(bytes(u), dtype=np.uint8), np.uint8)
+        v.cdef_idx = put(frame.cdef_idx, np.int8)
+        self.v = v
+
+
 class Server:
     """ctypes face of oxav1_pf_server_*."""
 
@@ -132,6 +194,19 @@ class Server:
         abi.check(st, "server_submit")
         return t.value
 
+    def submit_host(self, sid, desc, host, slot, arrival_us):
+        """Ticket, or None when the slot or every input buffer is busy."""
+        t = C.c_uint64()
+        st = self.lib.oxav1_pf_server_submit_host(self.h, sid, C.byref(desc), C.byref(host), slot, arrival_us,
+                                                  C.byref(t))
+        if st == abi.ERR_BUSY:
+            return None
+        abi.check(st, "server_submit_host")
+        return t.value
+
+    def frame_input_bytes(self, desc):
+        return int(self.lib.oxav1_pf_server_frame_input_bytes(C.byref(desc)))
+
     def submit_existing(self, sid, slot, fg, arrival_us):
         """show_existing_frame output of a slot with grain parameters fg;
         ticket, or None when the slot is still busy."""
@@ -175,7 +250,11 @@ class DecodeSession:
     a GenConfig is given), a pool of distinct pre-filter frames, Poisson
     arrivals at fps."""
 
-    def __init__(self, seed, n_frames, fps=30.0, pool=8, slots=8, gen_config=None):
+    def __init__(self, seed, n_frames, fps=30.0, pool=8, slots=8, gen_config=None, host=False, in_flight=3):
+        """host: the frames stay in page-locked host memory and every
+        submission copies one to the device (oxav1_pf_server_submit_host;
+        in_flight input buffers per session), and every result is copied
+        back to host memory; else the pool is uploaded once at admission."""
         lib = abi.load()
         if gen_config is None:
             gen_config = abi.GenConfig()
@@ -187,9 +266,22 @@ class DecodeSession:
         self.slots = slots
         self.desc = self.frames[0].desc
         self.pixels = int(self.desc.upscaled_width) * int(self.desc.frame_height)
-        layouts = [input_layout(f)[1] for f in self.frames]
-        self.input_offsets = np.cumsum([0] + layouts[:-1]).tolist()
-        self.input_bytes = int(sum(layouts))
+        self.host = host
+        if host:
+            self.host_frames = [HostFrame(f) for f in self.frames]
+            st = np.uint8 if self.desc.bit_depth == 8 else np.uint16
+            grain = bool(self.desc.fg.apply_grain)
+            self.host_ref = [[HostArray(self.frames[0].out_shape(p), st) for p in range(self.desc.num_planes)]
+                             for _ in range(slots)]
+            self.host_out = [[HostArray(self.frames[0].out_shape(p), st) for p in range(self.desc.num_planes)]
+                             if grain else None for _ in range(slots)]
+            per = int(lib.oxav1_pf_server_frame_input_bytes(C.byref(self.desc)))
+            self.input_bytes = per * in_flight
+            self.input_offsets = None
+        else:
+            layouts = [input_layout(f)[1] for f in self.frames]
+            self.input_offsets = np.cumsum([0] + layouts[:-1]).tolist()
+            self.input_bytes = int(sum(layouts))
         rng = random.Random(seed)
         t, self.offsets_us = 0.0, []
         for _ in range(n_frames):
@@ -219,7 +311,8 @@ class DecodeSession:
         if r is None:
             return False
         self.sid, base, self.footprint = r
-        self.devs = [upload_inputs(f, base + off) for f, off in zip(self.frames, self.input_offsets)]
+        if not self.host:
+            self.devs = [upload_inputs(f, base + off) for f, off in zip(self.frames, self.input_offsets)]
         return True
 
     def start(self, now_us):
@@ -228,6 +321,24 @@ class DecodeSession:
     def frame_job(self, k):
         j = k % len(self.frames)
         return self.frames[j].desc, self.devs[j], k % self.slots
+
+    def frame_job_host(self, k):
+        """(desc, FrameHost with this frame's inputs and its slot's host
+        output buffers, slot)."""
+        j, slot = k % len(self.frames), k % self.slots
+        v = abi.FrameHost.from_buffer_copy(self.host_frames[j].v)
+        for p in range(self.desc.num_planes):
+            v.ref_out[p] = self.host_ref[slot][p].ptr
+            if self.host_out[slot] is not None:
+                v.out[p] = self.host_out[slot][p].ptr
+        return self.frames[j].desc, v, slot
+
+    def submit(self, server, k, arrival_us):
+        if self.host:
+            desc, v, slot = self.frame_job_host(k)
+            return server.submit_host(self.sid, desc, v, slot, arrival_us)
+        desc, dev, slot = self.frame_job(k)
+        return server.submit(self.sid, desc, dev, slot, arrival_us)
 
     def held_frames(self):
         """(slot, pool index) of the frames the slots hold once every frame is done."""
@@ -283,6 +394,7 @@ def run_serving(server, sessions, expected=None, max_seconds=120.0):
     waiting = list(range(len(sessions)))
     events = []  # (arrival_us, session index, frame index)
     lat_ms = []
+    dev_ms = []
     queue_ms = []
     batches = []
     mismatches = []
@@ -326,6 +438,10 @@ def run_serving(server, sessions, expected=None, max_seconds=120.0):
                     if ok and s.desc.fg.apply_grain:
                         got = server.slot(s.sid, slot, p, 1, s.out_shape(p), s.sample_dtype)
                         ok = np.array_equal(got.astype(np.int32), out_e[p])
+                    if ok and s.host:  # the copies handed back to the host
+                        ok = np.array_equal(s.host_ref[slot][p].a.astype(np.int32), ref_e[p])
+                        if ok and s.host_out[slot] is not None:
+                            ok = np.array_equal(s.host_out[slot][p].a.astype(np.int32), out_e[p])
                     if not ok:
                         mismatches.append((i, slot, j, p))
                 verified += 1
@@ -347,8 +463,7 @@ def run_serving(server, sessions, expected=None, max_seconds=120.0):
             pend.append(heapq.heappop(events))
         for arr, i, k in pend:
             s = sessions[i]
-            desc, dev, slot = s.frame_job(k)
-            if server.submit(s.sid, desc, dev, slot, arr) is None:
+            if s.submit(server, k, arr) is None:
                 retry.append((arr, i, k))
                 continue
             if first_arrival is None:
@@ -362,6 +477,7 @@ def run_serving(server, sessions, expected=None, max_seconds=120.0):
             n_done += 1
             s.latency_ms.append((r.done_us - r.arrival_us) * 1e-3)
             lat_ms.append((r.done_us - r.arrival_us) * 1e-3)
+            dev_ms.append((r.done_us - r.launch_us) * 1e-3)
             queue_ms.append((r.launch_us - r.arrival_us) * 1e-3)
             batches.append(r.batch)
             done_px += s.pixels
@@ -387,6 +503,8 @@ def run_serving(server, sessions, expected=None, max_seconds=120.0):
         "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat[-1]),
         "queue_p99_ms": float(np.percentile(np.array(queue_ms), 99)) if queue_ms else 0.0,
+        "launch_to_done_p50_ms": float(np.percentile(np.array(dev_ms), 50)) if dev_ms else 0.0,
+        "launch_to_done_p99_ms": float(np.percentile(np.array(dev_ms), 99)) if dev_ms else 0.0,
         "mean_batch": float(np.mean(batches)) if batches else 0.0,
         "max_batch": int(max(batches)) if batches else 0,
         "verified_frames": verified,

@@ -84,9 +84,13 @@ def test_cfg3_mixed_batch_64_frames(oracle_mod):
             df.free()
 
 
-def test_cfg4_serving_32_sessions(oracle_mod):
+@pytest.mark.parametrize("host", [False, True], ids=["device_resident", "host_frames"])
+def test_cfg4_serving_32_sessions(host, oracle_mod):
+    """host_frames: every frame is copied from page-locked host memory when
+    it is submitted and its result copied back (the decoder's CPU hand-off)."""
     from oxav1.serving import DecodeSession, Server, expected_outputs, run_serving
-    sessions = [DecodeSession(0x5E4F00 + i, n_frames=2, fps=30.0, pool=2, slots=2) for i in range(32)]
+    sessions = [DecodeSession(0x5E4F00 + i, n_frames=2, fps=30.0, pool=2, slots=2, host=host)
+                for i in range(32)]
     kinds = {(s.desc.upscaled_width, s.desc.bit_depth) for s in sessions}
     assert {w for w, _ in kinds} == {1280, 1920, 3840}
     expected = expected_outputs(sessions)

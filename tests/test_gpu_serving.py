@@ -20,10 +20,10 @@ MIX = [
 ]
 
 
-def _sessions(cfgs, n_frames, fps, seed0=0x5E55):
+def _sessions(cfgs, n_frames, fps, seed0=0x5E55, host=False):
     from oxav1.serving import DecodeSession
     return [DecodeSession(seed0 + i, n_frames=n_frames, fps=fps, pool=n_frames, slots=n_frames,
-                          gen_config=gen_config(**kw)) for i, kw in enumerate(cfgs)]
+                          gen_config=gen_config(**kw), host=host) for i, kw in enumerate(cfgs)]
 
 
 def _assert_alone_is_oracle(sessions, expected, oracle_mod):
@@ -52,6 +52,29 @@ def test_serving_mix_equals_alone(oracle_mod):
     assert res["verified_frames"] == 4 * len(MIX)
     assert res["mismatches"] == []
     assert res["stats"]["frames_done"] == res["stats"]["frames_submitted"] == 4 * len(MIX)
+
+
+def test_serving_host_frames_mix(oracle_mod):
+    """The same mix with every frame in host memory: each submission copies
+    the frame to the device while other frames are being filtered, and the
+    results come back to host memory; bursty arrivals with two input
+    buffers per session, so submissions also wait for free buffers.  Every
+    host copy and every slot equals the frame filtered alone."""
+    from oxav1.serving import DecodeSession, Server, expected_outputs, run_serving
+    sessions = [DecodeSession(0x5E57 + i, n_frames=6, fps=3000.0, pool=3, slots=3,
+                              gen_config=gen_config(**kw), host=True, in_flight=2)
+                for i, kw in enumerate(MIX)]
+    expected = expected_outputs(sessions)
+    _assert_alone_is_oracle(sessions, expected, oracle_mod)
+    srv = Server(lanes=3, max_batch=6)
+    try:
+        res = run_serving(srv, sessions, expected, max_seconds=60)
+    finally:
+        srv.close()
+    print("serving host frames:", {k: v for k, v in res.items() if k != "stats"}, flush=True)
+    assert res["frames"] == 6 * len(MIX)
+    assert res["verified_frames"] == 3 * len(MIX)
+    assert res["mismatches"] == []
 
 
 def test_serving_admission_by_footprint():
