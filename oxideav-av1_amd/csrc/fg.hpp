@@ -441,11 +441,6 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
     const int32_t hw_old = psx ? 23 : (j == 0 ? 27 : 17), hw_new = psx ? 22 : (j == 0 ? 17 : 27);
     const G<int16_t> *pc = block_base(st, b) + j;
     const G<int16_t> *pl = hov ? block_base(st, b - 1) + j + (1 << bsh) : pc;
-    auto hval = [&](const G<int16_t> *c, const G<int16_t> *l, int32_t i) -> int32_t {
-        int32_t g = c[i * tw];
-        if (hov) g = clip3(gmin, gmax, fg_round2(l[i * tw] * hw_old + g * hw_new, 5));
-        return g;
-    };
     // vertical overlap: the first 2 (1) rows of a stripe blend with rows 32,
     // 33 (16) of the previous stripe's noise
     const int32_t vrows = (overlap && st > 0) ? (psy ? 1 : 2) : 0;
@@ -463,31 +458,77 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
     const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
     const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) * (1 << (bd - 8));
     const int32_t maxs = (1 << bd) - 1;
-    for (int32_t y = yb0, i = ib; y < yb1; y++, i++, src += ip, dst += op) {
-        int32_t nz = hval(pc, pl, i);
-        if (i < vrows) {
-            const int32_t old = hval(pu, pul, i + (1 << ssh));
-            const int32_t wo = psy ? 23 : (i == 0 ? 27 : 17), wn = psy ? 22 : (i == 0 ? 17 : 27);
-            nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
+    // The band is blended kFgRows rows at a time: every load of a chunk
+    // (template values, source samples, co-located luma) is issued before
+    // the first store, at rows clamped to the band, so a chunk costs one
+    // memory round trip instead of one per row.  The vertical-overlap rows
+    // (the first 2, or 1 when vertically subsampled, rows of a noise stripe)
+    // only start a band, so their previous-stripe values are loaded with
+    // every chunk (used by the first); every such load stays inside the
+    // template: block bases are at most 39 rows down in a 73-row template
+    // (21 in a 38-row one when subsampled), and the rows read are 32, 33
+    // (16) below them.
+    constexpr int kFgRows = 4;
+    const int32_t ovr = 1 << ssh;  // rows of a stripe's noise
+    // 32-bit element offsets from the (wave-uniform) plane and template
+    // bases, so each load needs one address register
+    const int32_t oc_ = (int32_t)(pc - tmpl), ol_ = (int32_t)(pl - tmpl);
+    const int32_t ou_ = (int32_t)(pu - tmpl), oul_ = (int32_t)(pul - tmpl);
+    const uint32_t so_ = (uint32_t)yb0 * ip + (uint32_t)x, do_ = (uint32_t)yb0 * op + (uint32_t)x;
+    for (int32_t yc = yb0; yc < yb1; yc += kFgRows) {
+        const int32_t n = min(kFgRows, yb1 - yc);
+        int32_t nzc[kFgRows], nzl[kFgRows], org[kFgRows], la[kFgRows], lb[kFgRows];
+#pragma unroll
+        for (int r = 0; r < kFgRows; r++) {
+            const int32_t dy = yc - yb0 + min(r, n - 1);
+            const int32_t i = ib + dy;
+            nzc[r] = tmpl[oc_ + i * tw];
+            nzl[r] = tmpl[ol_ + i * tw];
+            org[r] = (int32_t)in[so_ + (uint32_t)dy * ip];
+            if (plane) {
+                const uint32_t lo = (uint32_t)((yb0 + dy) << psy) * lp;
+                la[r] = (int32_t)ly[lo + (uint32_t)lx];
+                lb[r] = (int32_t)ly[lo + (uint32_t)lnx];
+            }
         }
-        const int32_t orig = (int32_t)*src;
-        if (plane == 0) {
-            const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
-            *dst = (T)clip3(minv, maxl, orig + n2);
-            continue;
+        int32_t oc[2], ol[2];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int32_t row = (psy ? 0 : r) + ovr;
+            oc[r] = tmpl[ou_ + row * tw];
+            ol[r] = tmpl[oul_ + row * tw];
         }
-        // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
-        const G<T> *lrow = ly + (size_t)(y << psy) * lp;
-        const int32_t avg = psx ? fg_round2((int32_t)lrow[lx] + (int32_t)lrow[lnx], 1) : (int32_t)lrow[lx];
-        int32_t merged;
-        if (from_luma) {
-            merged = avg;
-        } else {
-            const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
-            merged = clip3(0, maxs, (comb >> 6) + coff);
+#pragma unroll
+        for (int r = 0; r < kFgRows; r++) {
+            if (r >= n) break;
+            const int32_t dy = yc - yb0 + r, i = ib + dy;
+            int32_t nz = nzc[r];
+            if (hov) nz = clip3(gmin, gmax, fg_round2(nzl[r] * hw_old + nz * hw_new, 5));
+            if (i < vrows) {
+                int32_t old = oc[i];
+                if (hov) old = clip3(gmin, gmax, fg_round2(ol[i] * hw_old + old * hw_new, 5));
+                const int32_t wo = psy ? 23 : (i == 0 ? 27 : 17), wn = psy ? 22 : (i == 0 ? 17 : 27);
+                nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
+            }
+            const int32_t orig = org[r];
+            G<T> *d = out + (do_ + (uint32_t)dy * op);
+            if (plane == 0) {
+                const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
+                *d = (T)clip3(minv, maxl, orig + n2);
+                continue;
+            }
+            // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
+            const int32_t avg = psx ? fg_round2(la[r] + lb[r], 1) : la[r];
+            int32_t merged;
+            if (from_luma) {
+                merged = avg;
+            } else {
+                const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
+                merged = clip3(0, maxs, (comb >> 6) + coff);
+            }
+            const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
+            *d = (T)clip3(minv, maxc, orig + n2);
         }
-        const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
-        *dst = (T)clip3(minv, maxc, orig + n2);
     }
 }
 
