@@ -767,6 +767,82 @@ int oxav1_pf_fg_apply_grain(const oxav1_film_grain_params *fg, const uint16_t *c
                             uint32_t height, uint8_t bit_depth, uint8_t ssx, uint8_t ssy, uint8_t num_planes,
                             uint8_t matrix_coefficients, uint16_t *const out[3]);
 
+/* Loop-restoration election (encoder/lr_elect.rs). */
+
+/* LrElectInput (encoder/lr_elect.rs:95-131). Planes are host u16 arrays,
+ * row-major without padding: luma width x height, chroma chroma_w x
+ * chroma_h, which must be (width + subsampling_x) >> subsampling_x and
+ * (height + subsampling_y) >> subsampling_y. input is the source the
+ * filters are fitted to, curr the pre-CDEF reconstruction (§7.17 reads it
+ * across stripe boundaries), cdef the post-CDEF reconstruction (the
+ * restoration input). restoration_type_cdf is the frame-start
+ * RestorationType CDF the unit symbols are priced with ({9413, 22581,
+ * 32768, 0} by default, cdf.rs:3703): non-decreasing, first entry > 0, third
+ * 32768. disable_cdf_update, use_superres and superres_denom are carried
+ * for the caller's write-side window (lr_elect.rs:790-806); they do not
+ * change the election (every unit is priced from a fresh copy of the CDF). */
+typedef struct oxav1_lr_elect_input {
+    const uint16_t *input[3];
+    const uint16_t *curr[3];
+    const uint16_t *cdef[3];
+    uint32_t width, height, chroma_w, chroma_h;
+    uint8_t bit_depth, subsampling_x, subsampling_y, num_planes;
+    uint32_t mi_rows, mi_cols;
+    uint64_t lambda;
+    uint16_t restoration_type_cdf[4];
+    uint8_t disable_cdf_update, use_superres;
+    uint8_t pad[2];
+    uint32_t superres_denom;
+} oxav1_lr_elect_input;
+
+/* One unit's candidates as the election scored them (lr_elect.rs:444-700):
+ * the SSD against the source of the unfiltered unit, of the fitted Wiener
+ * filter and of each self-guided set with its fitted projection weights. */
+typedef struct oxav1_lr_unit_cands {
+    uint64_t d_none, d_wiener;
+    uint64_t d_sgr[16];
+    int16_t wiener[2][3];
+    int16_t sgr_xqd[16][2];
+} oxav1_lr_unit_cands;
+
+/* LrPlan (lr_elect.rs:71-93) with the header block of header_shape
+ * (lr_elect.rs:196-214): 64x64 units on every plane. elected = 0 is the
+ * reference's None (every unit elected RESTORE_NONE). units[p] and
+ * cands[p] are caller-owned arrays of unit_rows[p] x unit_cols[p] entries
+ * (raster order), either may be NULL; units receive every unit's elected
+ * filter (RESTORE_NONE units included), cands the per-unit candidate
+ * records. d is the whole-frame SSD after the plan, d_pre before it; both
+ * are filled whether or not a plan is elected. */
+typedef struct oxav1_lr_election {
+    int32_t elected;
+    uint8_t frame_restoration_type[3];
+    uint8_t uses_lr, uses_chroma_lr;
+    uint8_t pad[3];
+    uint32_t loop_restoration_size[3];
+    uint32_t unit_rows[3], unit_cols[3];
+    uint64_t d, d_pre;
+    oxav1_lr_unit *units[3];
+    oxav1_lr_unit_cands *cands[3];
+} oxav1_lr_election;
+
+/* Replaces elect_lr (encoder/lr_elect.rs:421): per plane and 64x64 unit,
+ * the alternating least-squares Wiener fit, the self-guided probe fits of
+ * the 16 parameter sets, the exact §7.17 evaluation of every candidate and
+ * the D*256 + lambda*R256 choice with the recentred-subexp rate model, in
+ * the reference's order and with its tie-breaking. The fits run on the GPU
+ * in the reference's double-precision operation order (one sequential sum
+ * per normal-equation term); the candidate evaluations are the LR kernels
+ * reducing per-unit SSD and probe sums instead of storing samples.
+ * Synchronous. */
+int oxav1_pf_lr_elect(const oxav1_lr_elect_input *in, oxav1_lr_election *out);
+
+/* apply_lr_plan (lr_elect.rs:829-912): one §7.17 run of the plan's units
+ * over recon (the post-CDEF planes, restored in place) with in->curr as
+ * the pre-CDEF planes, on the GPU. *ssd receives the whole-frame SSD of
+ * the restored planes against in->input (in->cdef is not read). */
+int oxav1_pf_lr_apply_plan(const oxav1_lr_election *plan, const oxav1_lr_elect_input *in, uint16_t *const recon[3],
+                           uint64_t *ssd);
+
 #ifdef __cplusplus
 }
 #endif

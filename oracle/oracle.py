@@ -126,6 +126,20 @@ _SIGS = {
     "oo_fg_apply_grain": (None, [_P(abi.FilmGrainParams), C.c_void_p * 3, C.c_uint32, C.c_uint32, C.c_uint8,
                                  C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8, C.c_void_p * 3]),
     "oo_cdef_run_units": (None, [_P(abi.CdefElectInput), _P(abi.CdefParams), C.c_int, _P(C.c_uint64)]),
+    "oo_lr_elect": (C.c_int, [_P(abi.LrElectInput), _P(abi.LrElection)]),
+    "oo_lr_apply_plan": (C.c_int, [_P(abi.LrElection), _P(abi.LrElectInput), C.c_void_p * 3, _P(C.c_uint64)]),
+    "oo_lr_unit_rects": (C.c_uint32, [_P(abi.LrElectInput), C.c_uint8, C.c_uint32, C.c_uint32, _P(C.c_uint32),
+                                      C.c_uint32]),
+    "oo_lr_fit_wiener": (None, [_P(C.c_int32), _P(C.c_uint16), C.c_uint32, C.c_uint32, _P(C.c_uint32), C.c_uint32,
+                                C.c_int, _P(C.c_int32), _P(C.c_double)]),
+    "oo_log2_frac256": (C.c_uint32, [C.c_uint32]),
+    "oo_recenter": (C.c_int64, [C.c_int64, C.c_int64]),
+    "oo_cw_cost": (C.c_uint64, [C.c_uint32, C.c_int, C.c_int64, C.c_int64, C.c_uint32, C.c_int64, C.c_int64,
+                                _P(C.c_uint16)]),
+    "oo_cw_symbol_run": (C.c_uint64, [C.c_uint32, C.c_uint32, _P(C.c_uint16), C.c_uint32, C.c_uint32]),
+    "oo_cw_signed_ref_bits": (C.c_uint32, [C.c_int64, C.c_int64, C.c_uint32, C.c_int64, C.c_int64,
+                                           _P(C.c_uint8), C.c_uint32, _P(C.c_uint64)]),
+    "oo_lr_price_unit": (C.c_uint64, [_P(C.c_int32), _P(C.c_int32), C.c_int, _P(abi.LrUnit), _P(C.c_uint16)]),
 }
 
 _lib = None

@@ -958,7 +958,7 @@ static void lr_wiener(const oxav1_lr_ctx *ctx, const oxav1_lr_unit *u, const oxa
 }
 
 /* loop_restore_block (loop_restoration.rs:430-460). */
-static void lr_restore_block(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *g,
+void oo_lr_restore_block(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *g,
                              const oxav1_plane_i32 *curr, uint32_t ncurr,
                              const oxav1_plane_i32 *cdef, uint32_t ncdef, oxav1_plane_i32 *lr,
                              uint32_t nlr, uint8_t plane, uint32_t row, uint32_t col) {
@@ -992,7 +992,7 @@ void oo_loop_restoration_frame(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *g,
         for (uint32_t x = 0; x < ctx->upscaled_width; x += MI_SIZE)
             for (uint32_t p = 0; p < np; p++) {
                 if (ctx->frame_restoration_type[p] == OXAV1_RESTORE_NONE) continue;
-                lr_restore_block(ctx, g, curr, ncurr, cdef, ncdef, lr, nlr, (uint8_t)p, y >> 2, x >> 2);
+                oo_lr_restore_block(ctx, g, curr, ncurr, cdef, ncdef, lr, nlr, (uint8_t)p, y >> 2, x >> 2);
             }
 }
 

@@ -75,6 +75,40 @@ int oo_cdef_elect(const oxav1_cdef_elect_input *in, oxav1_cdef_election *out);
 void oo_cdef_run_units(const oxav1_cdef_elect_input *in, const oxav1_cdef_params *params, int which,
                        uint64_t *ssd);
 
+/* ---- encoder loop-restoration election (encoder/lr_elect.rs), oo_lr_elect.c ---- */
+int oo_lr_elect(const oxav1_lr_elect_input *in, oxav1_lr_election *out);
+int oo_lr_apply_plan(const oxav1_lr_election *plan, const oxav1_lr_elect_input *in, uint16_t *const recon[3],
+                     uint64_t *ssd);
+/* The unit's output rects (x, y, w, h per rect, in push order); returns
+ * the count (at most cap are written). */
+uint32_t oo_lr_unit_rects(const oxav1_lr_elect_input *in, uint8_t plane, uint32_t ur, uint32_t uc,
+                          uint32_t *rects, uint32_t cap);
+/* fit_wiener over the rects; trace[s] = the fitted tap vector after step s
+ * (round 0 horizontal, round 0 vertical, round 1 horizontal, round 1
+ * vertical), taps = [vertical, horizontal] quantised. */
+void oo_lr_fit_wiener(const int32_t *cdef, const uint16_t *src, uint32_t plane_w, uint32_t plane_h,
+                      const uint32_t *rects, uint32_t nrects, int first_coeff, int32_t taps[2][3],
+                      double trace[4][3]);
+/* The counting symbol writer (encoder/symbol_writer.rs:168-345). */
+uint32_t oo_log2_frac256(uint32_t x);
+int64_t oo_recenter(int64_t r, int64_t v);
+/* cost256 of a symbol sequence from range: kind 0 = symbol (value) over
+ * cdf (n symbols), 1 = literal (n bits), 2 = subexp (num_syms = n, k),
+ * 3 = signed subexp with reference (low .. high = n, k, r) — one element. */
+uint64_t oo_cw_cost(uint32_t range, int kind, int64_t value, int64_t n, uint32_t k, int64_t low, int64_t r,
+                    const uint16_t *cdf);
+/* reps writes of one symbol over a fixed CDF, from range. */
+uint64_t oo_cw_symbol_run(uint32_t range, uint32_t symbol, const uint16_t *cdf, uint32_t n, uint32_t reps);
+/* The bools one signed subexp code writes (at most cap recorded), and its
+ * cost256 from range 0x8000. */
+uint32_t oo_cw_signed_ref_bits(int64_t low, int64_t high, uint32_t k, int64_t r, int64_t v, uint8_t *bits,
+                               uint32_t cap, uint64_t *cost);
+/* price_unit (lr_elect.rs:537-545) from the reference state ref_wiener
+ * [2][3] / ref_xqd[2] of the plane: the SWITCHABLE symbol and the unit's
+ * coefficients, cost256 from range 0x8000. */
+uint64_t oo_lr_price_unit(const int32_t ref_wiener[2][3], const int32_t ref_xqd[2], int plane,
+                          const oxav1_lr_unit *u, const uint16_t rt_cdf[4]);
+
 /* ---- encoder superres election helpers (encoder/superres_elect.rs), oo_elect.c ---- */
 uint32_t oo_superres_coded_width(uint32_t upscaled_width, uint32_t denom);
 uint32_t oo_superres_candidate_denoms(uint32_t width, uint32_t *out);
@@ -118,6 +152,9 @@ int32_t oo_lr_get_source_sample(const oxav1_plane_i32 *curr, uint32_t ncurr,
                                 const oxav1_plane_i32 *cdef, uint32_t ncdef, uint8_t plane,
                                 int32_t x, int32_t y, const oo_lr_geom *geom);
 void oo_wiener_coefficients(const int32_t coeff[3], int32_t filter[7]);
+void oo_lr_restore_block(const oxav1_lr_ctx *ctx, const oxav1_lr_grids *g, const oxav1_plane_i32 *curr,
+                         uint32_t ncurr, const oxav1_plane_i32 *cdef, uint32_t ncdef, oxav1_plane_i32 *lr,
+                         uint32_t nlr, uint8_t plane, uint32_t row, uint32_t col);
 /* box_filter (loop_restoration.rs:594-729); out has geom->h*geom->w entries. */
 void oo_lr_box_filter(const oxav1_plane_i32 *curr, uint32_t ncurr, const oxav1_plane_i32 *cdef,
                       uint32_t ncdef, uint8_t plane, const oo_lr_geom *geom, uint8_t bit_depth,

@@ -121,7 +121,8 @@ __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSme
     const uint32_t per_stripe = J.lr_tiles_x[p] * lr_chunks(p ? J.sy : 0);
     const uint32_t stripe = t / per_stripe, rest = t - stripe * per_stripe;
     const uint32_t chunk = rest / J.lr_tiles_x[p], tile = rest - chunk * J.lr_tiles_x[p];
-    lr_tile<T>(J, p, stripe, chunk, tile, S, plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]);
+    LrStore<T> sink{plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]};
+    lr_tile<T>(J, p, stripe, chunk, tile, S, sink);
 }
 
 template <typename T>
@@ -1436,10 +1437,14 @@ extern "C" size_t oxav1_pf_struct_size(const char *name) {
     if (n == "oxav1_cdef_ssd_frame") return sizeof(oxav1_cdef_ssd_frame);
     if (n == "oxav1_cdef_elect_input") return sizeof(oxav1_cdef_elect_input);
     if (n == "oxav1_cdef_election") return sizeof(oxav1_cdef_election);
+    if (n == "oxav1_lr_elect_input") return sizeof(oxav1_lr_elect_input);
+    if (n == "oxav1_lr_unit_cands") return sizeof(oxav1_lr_unit_cands);
+    if (n == "oxav1_lr_election") return sizeof(oxav1_lr_election);
     if (n == "oxav1_noise_estimate") return sizeof(oxav1_noise_estimate);
     return 0;
 }
 
 #include "dropin.inc"
 #include "elect.inc"
+#include "lr_elect.inc"
 #include "noalloc.inc"

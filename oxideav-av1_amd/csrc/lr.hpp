@@ -126,11 +126,28 @@ __host__ __device__ inline LrSgrConsts lr_sgr_consts_of(int32_t r, int32_t e) {
     return c;
 }
 
-// One tile: (plane, stripe, column tile) -> out (cropped frame coordinates
-// equal plane coordinates).  Writes rows [ys, ye] x cols [xs, xe].
+// Where a tile's restored samples go.  LrStore writes them to the output
+// plane; the encoder's election passes sinks that reduce them instead.
+// unit(): the tile's restoration unit (called once the unit is resolved);
+// put(): a copied or Wiener sample; sgr(): a self-guided sample with its
+// filter outputs (u = the CDEF sample, f0/f1 = the two passes' flt values,
+// u << 4 where a pass is off).
 template <typename T>
+struct LrStore {
+    G<T> *__restrict__ out;
+    uint32_t opitch;
+    __device__ __forceinline__ void unit(uint32_t, uint32_t, const oxav1_lr_unit &) {}
+    __device__ __forceinline__ void put(int32_t y, int32_t x, int32_t v) { out[(size_t)y * opitch + x] = (T)v; }
+    __device__ __forceinline__ void sgr(int32_t y, int32_t x, int32_t v, int32_t, int32_t, int32_t) {
+        out[(size_t)y * opitch + x] = (T)v;
+    }
+};
+
+// One tile: (plane, stripe, column tile) -> sink (cropped frame coordinates
+// equal plane coordinates).  Emits rows [ys, ye] x cols [xs, xe].
+template <typename T, class Sink>
 __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
-                        LrSmem<typename LdsOf<T>::type> &S, G<T> *__restrict__ out, uint32_t opitch) {
+                        LrSmem<typename LdsOf<T>::type> &S, Sink &sink) {
     using L = typename LdsOf<T>::type;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
     LrTileGeom g;
@@ -155,8 +172,15 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
     const uint32_t dpitch = J.up_cdef_pitch[plane];
 
-    // unit of this tile (loop_restoration.rs:395-405)
-    uint32_t type = J.uses_lr ? J.lr_type[plane] : OXAV1_RESTORE_NONE;
+    // unit of this tile (loop_restoration.rs:395-405).  lr_type is read as
+    // the aligned dword holding lr_type[0..2]: a byte load indexed by the
+    // plane lets the compiler form the address J + plane and reuse it as the
+    // base of scalar loads of the dword arrays below (J + plane + 3 * plane +
+    // offset), and a scalar load ignores the low two bits of its base, so
+    // those loads would read another plane's fields.
+    uint32_t types;
+    __builtin_memcpy(&types, J.lr_type, sizeof types);
+    uint32_t type = J.uses_lr ? (types >> (8 * plane)) & 0xffu : OXAV1_RESTORE_NONE;
     oxav1_lr_unit u{};
     if (type != OXAV1_RESTORE_NONE) {
         const uint32_t us = J.lr_unit_size[plane];
@@ -164,6 +188,7 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
         const uint32_t uc = min(J.lr_unit_cols[plane] - 1, (uint32_t)g.xs / us);
         u = J.lr_units[plane] ? J.lr_units[plane][(size_t)ur * J.lr_unit_cols[plane] + uc] : oxav1_lr_unit{};
         type = u.type;
+        sink.unit(ur, uc, u);
     }
     // this chunk's rows of the stripe
     g.ys += (int32_t)chunk * kLrMaxH;
@@ -185,7 +210,7 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
                     v[i] = cdf[(size_t)(g.ys + min(rb + i, b1 - 1)) * dpitch + (g.xs + x)];
 #pragma unroll
                 for (int i = 0; i < kLrBand; i++)
-                    if (rb + i < b1) out[(size_t)(g.ys + rb + i) * opitch + (g.xs + x)] = v[i];
+                    if (rb + i < b1) sink.put(g.ys + rb + i, g.xs + x, (int32_t)v[i]);
             }
         return;
     }
@@ -298,13 +323,12 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
             int32_t h[kLrBand + 6];
 #pragma unroll
             for (int i = 0; i < kLrBand + 6; i++) h[i] = rb + i < H + 6 ? iv[(rb + i) * kLrMaxW] : 0;
-            G<T> *o = out + (size_t)(g.ys + rb) * opitch + (g.xs + lane);
 #pragma unroll
             for (int i = 0; i < kLrBand; i++)
                 if (rb + i < b1) {
                     const int32_t acc = lr_mul24(vf[0], h[i] + h[i + 6]) + lr_mul24(vf[1], h[i + 1] + h[i + 5]) +
                                         lr_mul24(vf[2], h[i + 2] + h[i + 4]) + lr_mul24(vf[3], h[i + 3]);
-                    o[(size_t)i * opitch] = (T)clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1);
+                    sink.put(g.ys + rb + i, g.xs + lane, clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1));
                 }
         }
         return;
@@ -421,7 +445,6 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     constexpr bool k24 = std::is_same<L, uint16_t>::value;
     auto mul = [](int32_t x, int32_t y) -> int32_t { return k24 ? lr_mul24(x, y) : x * y; };
     for (int kb = b0; kb < b1; kb += kLrBand) {
-        G<T> *o = out + (size_t)(ys + kb) * opitch + (g.xs + lane);
 #pragma unroll
         for (int i = 0; i < kLrBand; i++) {
             const int k = kb + i;
@@ -463,7 +486,7 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
             v += mul(w0, r0 != 0 ? f0 : u4);
             v += mul(w2, r1 != 0 ? f1 : u4);
             const int32_t sres = (v + (1 << 10)) >> 11;
-            o[(size_t)i * opitch] = (T)clip3(0, maxv, sres);
+            sink.sgr(ys + k, g.xs + lane, clip3(0, maxv, sres), uval, f0, f1);
         }
     }
 }

@@ -226,6 +226,29 @@ class CdefElection(C.Structure):
     ]
 
 
+class LrElectInput(C.Structure):
+    _fields_ = [
+        ("input", C.c_void_p * 3), ("curr", C.c_void_p * 3), ("cdef", C.c_void_p * 3),
+        ("width", u32), ("height", u32), ("chroma_w", u32), ("chroma_h", u32),
+        ("bit_depth", u8), ("subsampling_x", u8), ("subsampling_y", u8), ("num_planes", u8),
+        ("mi_rows", u32), ("mi_cols", u32), ("lambda_", u64), ("restoration_type_cdf", C.c_uint16 * 4),
+        ("disable_cdf_update", u8), ("use_superres", u8), ("pad", u8 * 2), ("superres_denom", u32),
+    ]
+
+
+class LrUnitCands(C.Structure):
+    _fields_ = [("d_none", u64), ("d_wiener", u64), ("d_sgr", u64 * 16), ("wiener", (i16 * 3) * 2),
+                ("sgr_xqd", (i16 * 2) * 16)]
+
+
+class LrElection(C.Structure):
+    _fields_ = [
+        ("elected", C.c_int32), ("frame_restoration_type", u8 * 3), ("uses_lr", u8), ("uses_chroma_lr", u8),
+        ("pad", u8 * 3), ("loop_restoration_size", u32 * 3), ("unit_rows", u32 * 3), ("unit_cols", u32 * 3),
+        ("d", u64), ("d_pre", u64), ("units", C.c_void_p * 3), ("cands", C.c_void_p * 3),
+    ]
+
+
 class NoiseEstimate(C.Structure):
     _fields_ = [("sigma16", u64), ("bin_sigma16", u64 * 8), ("cb_sigma16", u64), ("cb_bin_sigma16", u64 * 4),
                 ("cr_sigma16", u64), ("cr_bin_sigma16", u64 * 4)]
@@ -240,7 +263,8 @@ ABI_STRUCTS = {
     "oxav1_tuning": Tuning, "oxav1_walker_grids": WalkerGrids, "oxav1_server_config": ServerConfig, "oxav1_server_done": ServerDone,
     "oxav1_server_stats": ServerStats, "oxav1_cdef_cand": CdefCand, "oxav1_cdef_ssd_frame": CdefSsdFrame,
     "oxav1_cdef_elect_input": CdefElectInput, "oxav1_cdef_election": CdefElection,
-    "oxav1_noise_estimate": NoiseEstimate,
+    "oxav1_noise_estimate": NoiseEstimate, "oxav1_lr_elect_input": LrElectInput,
+    "oxav1_lr_unit_cands": LrUnitCands, "oxav1_lr_election": LrElection,
 }
 
 _P = C.POINTER
@@ -308,6 +332,8 @@ _SIGS = {
     "oxav1_pf_fg_plane_sigma16": (C.c_int, [C.c_void_p, C.c_void_p, u64, u8, _P(u64)]),
     "oxav1_pf_fg_apply_grain": (
         C.c_int, [_P(FilmGrainParams), C.c_void_p * 3, u32, u32, u8, u8, u8, u8, u8, C.c_void_p * 3]),
+    "oxav1_pf_lr_elect": (C.c_int, [_P(LrElectInput), _P(LrElection)]),
+    "oxav1_pf_lr_apply_plan": (C.c_int, [_P(LrElection), _P(LrElectInput), C.c_void_p * 3, _P(u64)]),
     "oxav1_pf_dev_alloc": (C.c_int, [_P(C.c_void_p), C.c_size_t]),
     "oxav1_pf_dev_free": (C.c_int, [C.c_void_p]),
     "oxav1_pf_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -264,3 +264,178 @@ def fg_apply_grain(fg, planes, bit_depth, ssx, ssy, matrix_coefficients=2, oracl
     if oracle is None:
         abi.check(st, "fg_apply_grain")
     return out
+
+
+# --- loop-restoration election (encoder/lr_elect.rs) --------------------------
+
+# Default_Restoration_Type_Cdf (cdf.rs:3703)
+DEFAULT_RESTORATION_TYPE_CDF = (9413, 22581, 32768, 0)
+
+
+@dataclass
+class LrElectInput:
+    input: list           # u16 planes: the source the filters are fitted to
+    curr: list            # u16 planes: pre-CDEF reconstruction
+    cdef: list            # u16 planes: post-CDEF reconstruction (the LR input)
+    bit_depth: int
+    subsampling_x: int
+    subsampling_y: int
+    lambda_: int = 0
+    restoration_type_cdf: tuple = DEFAULT_RESTORATION_TYPE_CDF
+    mi_rows: int = 0
+    mi_cols: int = 0
+    keep: list = field(default_factory=list)
+
+    @property
+    def num_planes(self):
+        return len(self.cdef)
+
+    def struct(self):
+        s = abi.LrElectInput()
+        arrs = []
+        for p in range(self.num_planes):
+            for name, src in (("input", self.input), ("curr", self.curr), ("cdef", self.cdef)):
+                a = np.ascontiguousarray(src[p], dtype=np.uint16)
+                arrs.append(a)
+                getattr(s, name)[p] = a.ctypes.data
+        h, w = self.cdef[0].shape
+        s.width, s.height = w, h
+        if self.num_planes > 1:
+            s.chroma_h, s.chroma_w = self.cdef[1].shape
+        s.bit_depth = self.bit_depth
+        s.subsampling_x, s.subsampling_y = self.subsampling_x, self.subsampling_y
+        s.num_planes = self.num_planes
+        s.mi_rows = self.mi_rows or (h + 3) // 4
+        s.mi_cols = self.mi_cols or (w + 3) // 4
+        s.lambda_ = self.lambda_
+        for i, v in enumerate(self.restoration_type_cdf):
+            s.restoration_type_cdf[i] = v
+        self.keep = arrs
+        return s
+
+
+def _unit_tuple(u):
+    return (u.type, u.sgr_set, tuple(tuple(r) for r in u.wiener), tuple(u.sgr_xqd))
+
+
+def _cands_dict(c):
+    return {"d_none": c.d_none, "d_wiener": c.d_wiener, "d_sgr": list(c.d_sgr),
+            "wiener": tuple(tuple(r) for r in c.wiener), "sgr_xqd": [tuple(x) for x in c.sgr_xqd]}
+
+
+@dataclass
+class LrElection:
+    elected: bool
+    frame_restoration_type: tuple
+    uses_lr: int
+    uses_chroma_lr: int
+    loop_restoration_size: tuple
+    unit_rows: tuple
+    unit_cols: tuple
+    d: int
+    d_pre: int
+    units: list           # per plane: list of (type, sgr_set, wiener, sgr_xqd) in raster order
+    cands: list           # per plane: list of candidate dicts
+    raw_units: list = field(default_factory=list)  # the ctypes unit arrays (for apply_lr_plan)
+
+    def struct(self):
+        s = abi.LrElection()
+        s.elected = int(self.elected)
+        for p in range(3):
+            s.frame_restoration_type[p] = self.frame_restoration_type[p]
+            s.loop_restoration_size[p] = self.loop_restoration_size[p]
+            s.unit_rows[p], s.unit_cols[p] = self.unit_rows[p], self.unit_cols[p]
+            if p < len(self.raw_units) and self.raw_units[p] is not None:
+                s.units[p] = C.addressof(self.raw_units[p])
+        s.uses_lr, s.uses_chroma_lr = self.uses_lr, self.uses_chroma_lr
+        s.d, s.d_pre = self.d, self.d_pre
+        return s
+
+
+def elect_lr(inp, lib=None, fn="oxav1_pf_lr_elect"):
+    """The loop-restoration election of one frame (elect_lr,
+    encoder/lr_elect.rs:421).  Unlike the reference's Option the result is
+    always returned; `elected` is False where the reference returns None.
+    `lib` may be the oracle's library (fn="oo_lr_elect")."""
+    lib = lib or abi.load()
+    s = inp.struct()
+    out = abi.LrElection()
+    units, cands = [], []
+    for p in range(inp.num_planes):
+        sy = inp.subsampling_y if p else 0
+        sx = inp.subsampling_x if p else 0
+        h, w = inp.cdef[p].shape
+        n = max((h + 32) // 64, 1) * max((w + 32) // 64, 1)
+        units.append((abi.LrUnit * n)())
+        cands.append((abi.LrUnitCands * n)())
+        out.units[p] = C.addressof(units[p])
+        out.cands[p] = C.addressof(cands[p])
+    st = getattr(lib, fn)(C.byref(s), C.byref(out))
+    if st:
+        raise abi.Oxav1Error(st, fn)
+    np_ = inp.num_planes
+    return LrElection(
+        bool(out.elected), tuple(out.frame_restoration_type), out.uses_lr, out.uses_chroma_lr,
+        tuple(out.loop_restoration_size), tuple(out.unit_rows), tuple(out.unit_cols), out.d, out.d_pre,
+        [[_unit_tuple(u) for u in units[p][:out.unit_rows[p] * out.unit_cols[p]]] for p in range(np_)],
+        [[_cands_dict(c) for c in cands[p][:out.unit_rows[p] * out.unit_cols[p]]] for p in range(np_)],
+        units)
+
+
+def apply_lr_plan(plan, inp, lib=None, fn="oxav1_pf_lr_apply_plan"):
+    """apply_lr_plan (lr_elect.rs:829): the plan over copies of inp.cdef;
+    returns (restored planes, whole-frame SSD against inp.input)."""
+    lib = lib or abi.load()
+    s = inp.struct()
+    ps = plan.struct()
+    recon = [np.ascontiguousarray(c, dtype=np.uint16).copy() for c in inp.cdef]
+    ssd = C.c_uint64(0)
+    st = getattr(lib, fn)(C.byref(ps), C.byref(s), _ptrs(recon), C.byref(ssd))
+    if st:
+        raise abi.Oxav1Error(st, fn)
+    return recon, ssd.value
+
+
+def detail_scene(x, y):
+    """Wide-band detail with no hard edges (the reference's LR content,
+    tests/lr_ab.rs:31-43)."""
+    return (120.0 + 34.0 * np.sin(0.13 * x) * np.cos(0.11 * y) + 22.0 * np.sin(0.31 * x + 0.17 * y)
+            + 14.0 * np.cos(0.53 * x - 0.29 * y) + 8.0 * np.sin(0.83 * x + 0.71 * y) + 0.15 * x + 0.1 * y)
+
+
+def synthetic_lr_input(seed, width, height, bit_depth=8, ssx=1, ssy=1, mono=False, lambda_=None, blur=1.0,
+                       noise=None, detail=True):
+    """An election input: a source of natural-looking detail (or noise
+    texture), a pre-CDEF reconstruction that is the source blurred with
+    quantisation-like noise, and a post-CDEF plane that smooths it a little
+    further."""
+    rng = np.random.default_rng(seed)
+    mx = (1 << bit_depth) - 1
+    sc = 1 << (bit_depth - 8)
+    noise = 2.0 * sc if noise is None else noise
+    dx, dy = rng.uniform(0, 50, 2)
+
+    def blur3(a, k):
+        p = np.pad(a, 1, mode="edge")
+        return (k * p[1:-1, 1:-1] + (1 - k) / 4 * (p[:-2, 1:-1] + p[2:, 1:-1] + p[1:-1, :-2] + p[1:-1, 2:]))
+
+    def plane(h, w, step, scale, off):
+        y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+        if detail:
+            base = off + scale * (detail_scene(x * step + dx, y * step + dy) - 120.0)
+        else:
+            base = off + rng.normal(0, 20, (h, w))
+        src = np.clip(np.rint(base * sc), 0, mx)
+        rec = blur3(src, 1.0 - 0.6 * blur) + rng.normal(0, noise, (h, w))
+        curr = np.clip(np.rint(rec), 0, mx)
+        cdef = np.clip(np.rint(blur3(curr, 0.8)), 0, mx)
+        return src.astype(np.uint16), curr.astype(np.uint16), cdef.astype(np.uint16)
+
+    pl = [plane(height, width, 1.0, 1.0, 120.0)]
+    if not mono:
+        ch, cw = (height + ssy) >> ssy, (width + ssx) >> ssx
+        pl += [plane(ch, cw, 2.0, 0.4, 128.0), plane(ch, cw, 2.0, -0.35, 128.0)]
+    if lambda_ is None:
+        lambda_ = int(rng.integers(20, 2000)) * sc * sc
+    return LrElectInput([p[0] for p in pl], [p[1] for p in pl], [p[2] for p in pl], bit_depth,
+                        ssx if not mono else 1, ssy if not mono else 1, lambda_)

@@ -13,7 +13,7 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SUITES = ["test_oracle_kat.py", "test_oracle_pins.py", "test_oracle_selfcheck.py",
-          "test_oracle_lf_edges.py", "test_oracle_elect.py", "test_tables.py"]
+          "test_oracle_lf_edges.py", "test_oracle_elect.py", "test_oracle_lr_elect.py", "test_tables.py"]
 
 
 def test_oracle_suites_under_ubsan():
