@@ -405,9 +405,39 @@ def elect_bench(reps=5, cpu=True):
                 else:
                     row["equal"][name] = r == want
         helpers.append(row)
+    # loop-restoration election (encoder/lr_elect.rs), host u16 planes in
+    lr_rows = []
+    for w, h in ((640, 360), (1920, 1080), (3840, 2160)):
+        inp = E.synthetic_lr_input(0x1E + w, w, h, 10, 1, 1, lambda_=600 * 16)
+        r = E.elect_lr(inp)
+        call = E.lr_elect_call(inp)
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            call()
+            lat.append((time.perf_counter() - t0) * 1e3)
+        row = {"frame": f"{w}x{h} 10-bit 4:2:0", "units": sum(len(u) for u in r.units),
+               "elect_ms": {"p50": round(statistics.median(lat), 2), "min": round(min(lat), 2)},
+               "elected": r.elected, "frame_restoration_type": list(r.frame_restoration_type),
+               "d": r.d, "d_pre": r.d_pre}
+        if cpu and w == 640:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            t0 = time.perf_counter()
+            want = E.elect_lr(inp, oracle.load(), "oo_lr_elect")
+            cpu_s = time.perf_counter() - t0
+            row["cpu_oracle"] = {"s": round(cpu_s, 2), "cores": 1, "kind": "port",
+                                 "speedup": round(cpu_s * 1e3 / statistics.median(lat), 1)}
+            row["equal_to_oracle"] = (want.units == r.units and want.cands == r.cands and want.d == r.d
+                                      and want.d_pre == r.d_pre and want.elected == r.elected)
+        lr_rows.append(row)
     return {"what": "encoder CDEF strength election (sweeps, damping refinement, greedy per-unit sets, "
                     "D*256 + lambda*R256), host u16 planes in, elected plan out",
             "frames": out,
+            "lr_what": "encoder loop-restoration election (Wiener and self-guided fits per 64x64 unit, "
+                       "exact evaluation of 18 candidates per unit, D*256 + lambda*R256), host u16 planes in, "
+                       "plan and per-unit candidate records out",
+            "lr_frames": lr_rows,
             "helpers_what": "superres and film-grain election helpers (superres_elect.rs, film_grain_elect.rs), "
                             "host u16 planes in and out (PCIe-inclusive), superres denominator 12",
             "helpers": helpers}
@@ -446,7 +476,11 @@ def main():
                     help="time the side paths on one 3840x2160 10-bit frame: side information built on the "
                          "GPU from walker grids, and the show_existing_frame grain output of a resident slot")
     ap.add_argument("--elect", action="store_true",
-                    help="time the encoder's CDEF strength election on 1080p and 4K 10-bit frames")
+                    help="time the encoder's CDEF and loop-restoration elections and the superres and film-grain "
+                         "election helpers on 1080p and 4K 10-bit frames")
+    ap.add_argument("--elect-lr", action="store_true",
+                    help="only the loop-restoration election's raw C call at 1080p and 4K (one warm call, "
+                         "--steps timed calls each), for profiling")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run only the plain device copy kernel over this many bytes, --steps times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters against a known traffic)")
@@ -492,6 +526,22 @@ def main():
         return
     if args.elect:
         print(json.dumps(elect_bench(cpu=not args.no_cpu_baseline)))
+        return
+    if args.elect_lr:
+        from oxav1 import elect as E
+        rows = []
+        for w, h in ((1920, 1080), (3840, 2160)):
+            inp = E.synthetic_lr_input(0x1E + w, w, h, 10, 1, 1, lambda_=600)
+            call = E.lr_elect_call(inp)
+            call()
+            lat = []
+            for _ in range(args.steps):
+                t0 = time.perf_counter()
+                call()
+                lat.append((time.perf_counter() - t0) * 1e3)
+            rows.append({"frame": f"{w}x{h} 10-bit 4:2:0", "calls": args.steps,
+                         "elect_ms_p50": round(statistics.median(lat), 2)})
+        print(json.dumps({"what": "oxav1_pf_lr_elect raw call, host u16 planes in, plan out", "rows": rows}))
         return
 
     seeds = workload_seeds(args.cfg, args.frames)

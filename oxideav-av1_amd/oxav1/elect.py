@@ -352,27 +352,36 @@ class LrElection:
         return s
 
 
-def elect_lr(inp, lib=None, fn="oxav1_pf_lr_elect"):
-    """The loop-restoration election of one frame (elect_lr,
-    encoder/lr_elect.rs:421).  Unlike the reference's Option the result is
-    always returned; `elected` is False where the reference returns None.
-    `lib` may be the oracle's library (fn="oo_lr_elect")."""
+def lr_elect_call(inp, lib=None, fn="oxav1_pf_lr_elect"):
+    """The raw call: returns a closure that runs the election into
+    preallocated ctypes arrays and returns (out, units, cands)."""
     lib = lib or abi.load()
     s = inp.struct()
     out = abi.LrElection()
     units, cands = [], []
     for p in range(inp.num_planes):
-        sy = inp.subsampling_y if p else 0
-        sx = inp.subsampling_x if p else 0
         h, w = inp.cdef[p].shape
         n = max((h + 32) // 64, 1) * max((w + 32) // 64, 1)
         units.append((abi.LrUnit * n)())
         cands.append((abi.LrUnitCands * n)())
         out.units[p] = C.addressof(units[p])
         out.cands[p] = C.addressof(cands[p])
-    st = getattr(lib, fn)(C.byref(s), C.byref(out))
-    if st:
-        raise abi.Oxav1Error(st, fn)
+    f = getattr(lib, fn)
+
+    def run():
+        st = f(C.byref(s), C.byref(out))
+        if st:
+            raise abi.Oxav1Error(st, fn)
+        return out, units, cands
+    return run
+
+
+def elect_lr(inp, lib=None, fn="oxav1_pf_lr_elect"):
+    """The loop-restoration election of one frame (elect_lr,
+    encoder/lr_elect.rs:421).  Unlike the reference's Option the result is
+    always returned; `elected` is False where the reference returns None.
+    `lib` may be the oracle's library (fn="oo_lr_elect")."""
+    out, units, cands = lr_elect_call(inp, lib, fn)()
     np_ = inp.num_planes
     return LrElection(
         bool(out.elected), tuple(out.frame_restoration_type), out.uses_lr, out.uses_chroma_lr,
