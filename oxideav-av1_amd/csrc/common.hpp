@@ -136,6 +136,17 @@ __device__ __forceinline__ T gread(const T *p, size_t i) {
     return ((const G<T> *)p)[i];
 }
 
+// A value every lane of the wave holds, moved to scalar registers.  FrameJob
+// fields arrive in vector registers (they are read with vector loads), so
+// without this the compiler forms every address derived from them per lane in
+// 64-bit vector arithmetic instead of a scalar base plus a 32-bit offset.
+__device__ __forceinline__ uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T>
+__device__ __forceinline__ T *wave_ptr(T *p) {
+    const uint64_t v = (uint64_t)p;
+    return (T *)(((uint64_t)wave_u32((uint32_t)(v >> 32)) << 32) | wave_u32((uint32_t)v));
+}
+
 template <typename T>
 __device__ __forceinline__ G<T> *plane_ptr(void *p) {
     return (G<T> *)p;

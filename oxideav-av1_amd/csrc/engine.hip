@@ -110,8 +110,10 @@ __device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local) {
     const uint32_t ip = deb ? J.deb_pitch[p] : J.cdf_pitch[p];
     G<T> *out = plane_ptr<T>(deb ? J.up_deb[p] : J.up_cdef[p]);
     const uint32_t op = deb ? J.up_deb_pitch[p] : J.up_cdef_pitch[p];
-    sr_rows<T>(in, ip, (int32_t)J.pad_cols[p], out, op, (int32_t)J.out_cols[p], y0, nrows, (int32_t)J.out_rows[p],
-               J.sr_step[p], J.sr_init[p], J.sr_maxx[p], J.bd);
+    sr_rows<T>(wave_ptr(in), wave_u32(ip), (int32_t)wave_u32(J.pad_cols[p]), wave_ptr(out), wave_u32(op),
+               (int32_t)wave_u32(J.out_cols[p]), (int32_t)wave_u32((uint32_t)y0), (int32_t)wave_u32((uint32_t)nrows),
+               (int32_t)wave_u32(J.out_rows[p]), (int32_t)wave_u32((uint32_t)J.sr_step[p]),
+               (int32_t)wave_u32((uint32_t)J.sr_init[p]), (int32_t)wave_u32((uint32_t)J.sr_maxx[p]), (int32_t)wave_u32(J.bd));
 }
 
 template <typename T>

@@ -7,9 +7,7 @@
 //
 // A lane owns one output column for every row of the item: the source
 // position, phase and eight taps depend on the column only, so they are
-// computed once and the rows stream through with independent global loads
-// (no LDS staging, no barriers).  Consecutive lanes read overlapping,
-// nearly consecutive input samples and write consecutive output samples.
+// computed once and the rows stream through (no LDS staging, no barriers).
 //
 // In the chain the deblocked frame is upscaled only on the rows loop
 // restoration reads from it: the two rows above and below every stripe
@@ -27,6 +25,13 @@ __constant__ int16_t c_upscale_filter[64][8];  // filled from kUpscaleFilter at 
 // upscale_sample (superres.rs:422-445): source position
 // -(1<<14) + initialSubpelX + x*stepX, integer part px, phase from bits 8..13,
 // taps at Clip3(0, maxX, px+k-3) further clamped to the stored row.
+//
+// A wave owns 64 consecutive output columns.  Their eight taps span at most
+// 64 consecutive input samples (the upscale ratio is at least 9/8), so each
+// lane loads one input sample per row and the taps are gathered across the
+// wave with ds_bpermute: one global load per output sample instead of eight.
+// A wave whose taps span more than 64 samples (only very narrow planes,
+// where the rounded downscaled width lifts the ratio) loads every tap.
 template <typename T>
 __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ipitch, int32_t in_cols, G<T> *__restrict__ out,
                         uint32_t opitch, int32_t out_cols, int32_t y0, int32_t nrows, int32_t ylim, int32_t step,
@@ -35,7 +40,10 @@ __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ip
     const int32_t last = min(maxx, in_cols - 1);  // taps never reach past this column
     const int32_t ya = max(y0, 0), yb = min(y0 + nrows, ylim);
     if (ya >= yb) return;
-    for (int32_t x = threadIdx.x; x < out_cols; x += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int32_t w0 = (int32_t)(threadIdx.x & ~63u);
+    for (int32_t xb = w0; xb < out_cols; xb += (int32_t)blockDim.x) {
+        const int32_t x = xb + lane;
         const int32_t src_x = -(1 << 14) + init + x * step;
         const int32_t px = src_x >> 14;
         const int32_t ph = (src_x & ((1 << 14) - 1)) >> 8;
@@ -45,25 +53,51 @@ __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ip
             f[k] = c_upscale_filter[ph][k];
             col[k] = clip3(0, last, clip3(0, maxx, px + (k - 3)));
         }
-        // rows in groups of four: the 32 tap loads of a group are
-        // unconditional (rows clamped into the item) so they overlap
-        G<T> *rout = out + (size_t)ya * opitch + x;
-        for (int32_t y = ya; y < yb; y += 4, rout += 4 * opitch) {
-            int32_t v[4][8];
+        // the wave's first and last tap columns (lanes past out_cols included:
+        // they gather like the others and store nothing)
+        const int32_t base = __builtin_amdgcn_readfirstlane(col[0]);
+        const int32_t top = __builtin_amdgcn_readlane(col[7], 63);
+        const bool store = x < out_cols;
+        if (top - base < 64) {
+            const uint32_t lc = (uint32_t)min(base + lane, last) * (uint32_t)sizeof(T);
+            int32_t sel[8];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const G<T> *row = in + (size_t)min(y + r, yb - 1) * ipitch;
+            for (int k = 0; k < 8; k++) sel[k] = (col[k] - base) << 2;
+            // four rows at a time, the next four loaded before these are
+            // filtered (rows clamped into the item, so every load is valid)
+            auto load4 = [&](int32_t y, int32_t *v) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) v[r][k] = (int32_t)row[col[k]];
+                for (int r = 0; r < 4; r++) {
+                    const G<uint8_t> *row = (const G<uint8_t> *)(in + (size_t)min(y + r, yb - 1) * ipitch);
+                    v[r] = (int32_t)*(const G<T> *)(row + lc);
+                }
+            };
+            int32_t v[4], vn[4];
+            load4(ya, v);
+            for (int32_t y = ya; y < yb; y += 4) {
+                if (y + 4 < yb) load4(y + 4, vn);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (y + r >= yb) break;
+                    int32_t sum = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) sum += __builtin_amdgcn_ds_bpermute(sel[k], v[r]) * f[k];
+                    if (store) {
+                        G<uint8_t> *orow = (G<uint8_t> *)(out + (size_t)(y + r) * opitch);
+                        *(G<T> *)(orow + (uint32_t)x * (uint32_t)sizeof(T)) = (T)clip3(0, maxv, (sum + 64) >> 7);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = vn[r];
             }
+            continue;
+        }
+        for (int32_t y = ya; y < yb; y++) {
+            const G<T> *row = in + (size_t)y * ipitch;
+            int32_t sum = 0;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                if (y + r >= yb) break;
-                int32_t sum = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) sum += v[r][k] * f[k];
-                rout[(size_t)r * opitch] = (T)clip3(0, maxv, (sum + 64) >> 7);
-            }
+            for (int k = 0; k < 8; k++) sum += (int32_t)row[col[k]] * f[k];
+            if (store) out[(size_t)y * opitch + x] = (T)clip3(0, maxv, (sum + 64) >> 7);
         }
     }
 }
