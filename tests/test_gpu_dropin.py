@@ -209,6 +209,13 @@ def test_frame_driver_through_dropin_calls(case, gpu_calls, oracle_mod):
     ref_o, out_o, _ = oracle_mod.chain(frame)
     _eq(ref_g, ref_o, "pre-grain reference")
     _eq(out_g, out_o, "film-grain output")
+    # the driver's planes reused over two frames (their previous contents
+    # must not leak into the result)
+    bufs = dropin.driver_buffers(frame)
+    for rep in range(2):
+        ref_b, out_b = dropin.frame_driver_chain(gpu_calls, frame, bufs=bufs)
+        _eq(ref_b, ref_o, f"pre-grain reference, reused planes {rep}")
+        _eq(out_b, out_o, f"film-grain output, reused planes {rep}")
 
 
 @pytest.mark.parametrize("case", __import__("lf_edge_cases").all_cases(), ids=lambda c: c.name)
