@@ -66,8 +66,10 @@ struct FrameJob {
     uint32_t sr_in_w[3];        // downscaled plane width (input columns used)
 
     // ---- loop restoration ----
-    uint8_t lr_type[3];
-    uint8_t pad3;
+    union {
+        uint8_t lr_type[4];     // per plane; [3] unused
+        uint32_t lr_type4;      // the four bytes as one aligned word
+    };
     uint32_t lr_unit_size[3], lr_unit_rows[3], lr_unit_cols[3];
     const oxav1_lr_unit *lr_units[3];
     uint32_t lr_tile_w[3];      // LR tile width (divides the unit size)

@@ -124,7 +124,11 @@ __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSme
     const uint32_t stripe = t / per_stripe, rest = t - stripe * per_stripe;
     const uint32_t chunk = rest / J.lr_tiles_x[p], tile = rest - chunk * J.lr_tiles_x[p];
     LrStore<T> sink{plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]};
-    lr_tile<T>(J, p, stripe, chunk, tile, S, sink);
+    // the job record is read-only for the whole launch: through the constant
+    // address space its fields are scalar loads (every base 4-byte aligned:
+    // lr_tile reads no byte array indexed by the plane)
+    using JC = const __attribute__((address_space(4))) FrameJob;
+    lr_tile<T, LrStore<T>, JC>(*(JC *)&J, p, stripe, chunk, tile, S, sink);
 }
 
 template <typename T>

@@ -144,9 +144,11 @@ struct LrStore {
 };
 
 // One tile: (plane, stripe, column tile) -> sink (cropped frame coordinates
-// equal plane coordinates).  Emits rows [ys, ye] x cols [xs, xe].
-template <typename T, class Sink>
-__device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
+// equal plane coordinates).  Emits rows [ys, ye] x cols [xs, xe].  JT is the
+// job record's type: the engine passes it in the constant address space, so
+// its fields (uniform in the workgroup) become scalar loads.
+template <typename T, class Sink, class JT = const FrameJob>
+__device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, uint32_t chunk, uint32_t tile,
                         LrSmem<typename LdsOf<T>::type> &S, Sink &sink) {
     using L = typename LdsOf<T>::type;
     const uint32_t sx = plane ? J.sx : 0, sy = plane ? J.sy : 0;
@@ -173,13 +175,12 @@ __device__ __forceinline__ void lr_tile(const FrameJob &J, uint32_t plane, uint3
     const uint32_t dpitch = J.up_cdef_pitch[plane];
 
     // unit of this tile (loop_restoration.rs:395-405).  lr_type is read as
-    // the aligned dword holding lr_type[0..2]: a byte load indexed by the
+    // the aligned word holding lr_type[0..2]: a byte load indexed by the
     // plane lets the compiler form the address J + plane and reuse it as the
     // base of scalar loads of the dword arrays below (J + plane + 3 * plane +
     // offset), and a scalar load ignores the low two bits of its base, so
     // those loads would read another plane's fields.
-    uint32_t types;
-    __builtin_memcpy(&types, J.lr_type, sizeof types);
+    const uint32_t types = J.lr_type4;
     uint32_t type = J.uses_lr ? (types >> (8 * plane)) & 0xffu : OXAV1_RESTORE_NONE;
     oxav1_lr_unit u{};
     if (type != OXAV1_RESTORE_NONE) {
