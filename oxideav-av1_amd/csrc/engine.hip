@@ -66,18 +66,13 @@ __device__ __forceinline__ uint32_t plane_of(const uint32_t *base, uint32_t npla
 }
 
 // --- stage bodies (templated on the sample type) ---
-struct LfShared {
-    uint16_t dec0[kLfCellRows * kLfEdges];
-    uint16_t dec1[kLfEdges * (kLfTile / 4)];
-};
-
 template <typename T>
 __device__ __forceinline__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *region, LfShared &sh) {
     const uint32_t p = plane_of(J.item_base[kStageLf], 3, local);
     const uint32_t t = local - J.item_base[kStageLf][p];
     const uint32_t ty = t / J.lf_tiles_x[p], tx = t - ty * J.lf_tiles_x[p];
     lf_tile<T>(J, p, ty, tx, plane_ptr<T>(J.src[p]), J.src_pitch[p], plane_ptr<T>(J.deb[p]), J.deb_pitch[p], region,
-               sh.dec0, sh.dec1);
+               sh);
 }
 
 template <typename T>
@@ -225,7 +220,7 @@ template <int KIND>
 __device__ __forceinline__ void lf_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter, uint32_t claim) {
     using L = typename std::conditional<KIND == 1, int32_t, uint16_t>::type;
-    __shared__ L region[kLfRegion * kLfStride];
+    __shared__ __align__(16) L region[kLfRegion * kLfStride];
     __shared__ LfShared sh;
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {

@@ -61,16 +61,35 @@ def device_info():
 class DeviceFrame:
     """One frame's inputs and outputs in HBM, plus its FrameDesc/FrameDev."""
 
-    def __init__(self, frame, fg_out=True):
+    def __init__(self, frame, fg_out=True, pad_pitch=False):
+        """pad_pitch: the caller's planes (input, pre-grain and grain output)
+        get a row pitch of the width rounded up to a multiple of 4 plus 4
+        samples, the padding filled with the largest sample value, as a
+        decoder's frame buffers with aligned rows would be."""
         self.frame = frame
         d = frame.desc
         self.desc = abi.FrameDesc.from_buffer_copy(d)
         st = np.uint8 if d.bit_depth == 8 else np.uint16
         self.sample_dtype = st
         np_ = d.num_planes
-        self.src = [DevBuffer(frame.planes[p].shape, st, frame.planes[p].astype(st)) for p in range(np_)]
-        self.ref_out = [DevBuffer(frame.out_shape(p), st) for p in range(np_)]
-        self.out = ([DevBuffer(frame.out_shape(p), st) for p in range(np_)]
+        self.cols = {}
+
+        def plane(shape, data=None):
+            rows, cols = shape
+            pitch = ((cols + 3) // 4) * 4 + 4 if pad_pitch else cols
+            if pitch == cols:
+                buf = DevBuffer(shape, st, None if data is None else data.astype(st))
+            else:
+                full = np.full((rows, pitch), (1 << d.bit_depth) - 1, dtype=st)
+                if data is not None:
+                    full[:, :cols] = data
+                buf = DevBuffer((rows, pitch), st, full)
+            self.cols[id(buf)] = cols
+            return buf
+
+        self.src = [plane(frame.planes[p].shape, frame.planes[p]) for p in range(np_)]
+        self.ref_out = [plane(frame.out_shape(p)) for p in range(np_)]
+        self.out = ([plane(frame.out_shape(p)) for p in range(np_)]
                     if (fg_out and d.fg.apply_grain) else None)
         self.lf_mi = DevBuffer(frame.lf_mi.shape, np.uint32, frame.lf_mi)
         self.delta_lf = DevBuffer(frame.delta_lf.shape, np.int8, frame.delta_lf)
@@ -107,8 +126,10 @@ class DeviceFrame:
 
     def result(self):
         """(ref_out, out) as numpy int32 planes (out = ref_out without grain)."""
-        ref = [b.download().astype(np.int32) for b in self.ref_out]
-        out = [b.download().astype(np.int32) for b in self.out] if self.out is not None else ref
+        def get(b):
+            return b.download()[:, :self.cols[id(b)]].astype(np.int32)
+        ref = [get(b) for b in self.ref_out]
+        out = [get(b) for b in self.out] if self.out is not None else ref
         return ref, out
 
     def output_pixels(self):

@@ -69,6 +69,35 @@ def test_chain_small_mixed(mode, oracle_mod):
     eng.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_chain_padded_pitch(mode, oracle_mod):
+    """Caller planes with a row pitch of the width rounded up to four
+    samples plus four, the padding at the largest sample value: deblocking
+    loads its input four samples per lane from such rows (padded planes are
+    a multiple of four samples wide), and the cropped outputs, often of odd
+    width, are written into them; nothing may read or write the padding."""
+    from oxav1 import Frame
+    from oxav1.engine import DeviceFrame, Engine
+    cfgs = [(gen_config(**kw), s) for kw, s in SMALL]
+    cfgs.append((gen_config(upscaled_width=226, frame_height=118, noise_amp=4), 7))  # chroma 114 wide
+    cfgs.append((gen_config(bit_depth=10, upscaled_width=198, frame_height=70, subsampling_x=0,
+                            subsampling_y=0, noise_amp=8), 8))
+    frames = [DeviceFrame(Frame.generate(c, s), pad_pitch=True) for c, s in cfgs]
+    assert any(df.frame.out_shape(p)[1] % 4 for df in frames for p in range(df.frame.num_planes))
+    eng = Engine(max_frames=len(frames))
+    if mode == 0:
+        for df in frames:
+            eng.prepare([df])
+            eng.run(0)
+            eng.sync()
+    else:
+        eng.prepare(frames)
+        eng.run(mode)
+        eng.sync()
+    _check(frames, oracle_mod, f"padded pitch mode {mode}")
+    eng.close()
+
+
 GEO_FORMATS = [  # (bit_depth, sx, sy, mono)
     (8, 1, 1, 0), (10, 1, 1, 0), (12, 1, 1, 0), (8, 1, 0, 0), (10, 0, 0, 0), (12, 1, 0, 0),
     (8, 1, 1, 1), (10, 0, 0, 0),
