@@ -72,6 +72,8 @@ struct FrameJob {
     };
     uint32_t lr_unit_size[3], lr_unit_rows[3], lr_unit_cols[3];
     const oxav1_lr_unit *lr_units[3];
+    // multiply-shift reciprocals (lr_mdiv) of lr_tiles_x and lr_unit_size
+    uint64_t lr_magic_tx[3], lr_magic_us[3];
     uint32_t lr_tile_w[3];      // LR tile width (divides the unit size)
     uint32_t lrc_rows[3], lrc_cols[3];  // extents of the LR "curr" buffer
     uint8_t lr_copy_none;       // tiles without restoration copy CDEF -> out

@@ -115,15 +115,15 @@ template <typename T>
 __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
     const uint32_t p = plane_of(J.item_base[kStageLr], 3, local);
     const uint32_t t = local - J.item_base[kStageLr][p];
-    const uint32_t per_stripe = J.lr_tiles_x[p] * lr_chunks(p ? J.sy : 0);
-    const uint32_t stripe = t / per_stripe, rest = t - stripe * per_stripe;
-    const uint32_t chunk = rest / J.lr_tiles_x[p], tile = rest - chunk * J.lr_tiles_x[p];
-    LrStore<T> sink{plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]};
     // the job record is read-only for the whole launch: through the constant
     // address space its fields are scalar loads (every base 4-byte aligned:
     // lr_tile reads no byte array indexed by the plane)
     using JC = const __attribute__((address_space(4))) FrameJob;
-    lr_tile<T, LrStore<T>, JC>(*(JC *)&J, p, stripe, chunk, tile, S, sink);
+    JC &JK = *(JC *)&J;
+    uint32_t stripe, chunk, tile;
+    lr_decode(JK, p, t, stripe, chunk, tile);
+    LrStore<T> sink{plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]};
+    lr_tile<T, LrStore<T>, JC>(JK, p, stripe, chunk, tile, S, sink);
 }
 
 template <typename T>
@@ -176,6 +176,9 @@ __device__ __forceinline__ void claim_items(uint32_t *counter, uint32_t total, u
     count = 0;
 }
 
+// The item loop of every stage kernel.  After thread 0 publishes a claim and
+// the workgroup barrier, every thread reads it before the body; the body's
+// trailing barrier orders those reads before thread 0 publishes the next one.
 #define OXAV1_ITEM_LOOP(BODY)                                                     \
     __shared__ uint32_t s_item, s_n;                                              \
     uint32_t base_ = 0, left_ = 0;                                                \
@@ -194,7 +197,6 @@ __device__ __forceinline__ void claim_items(uint32_t *counter, uint32_t total, u
                 __syncthreads();                                                  \
                 base_ = __builtin_amdgcn_readfirstlane(s_item);                   \
                 left_ = __builtin_amdgcn_readfirstlane(s_n ? s_n : 1u);           \
-                __syncthreads();                                                  \
             }                                                                     \
             item = base_++;                                                       \
             left_--;                                                              \
@@ -435,6 +437,9 @@ static int ensure_device_init() {
         for (int pass = 0; pass < 2; pass++)
             sc[set][pass] = lr_sgr_consts_of(kSgrParams[set * 4 + 2 * pass], kSgrParams[set * 4 + 2 * pass + 1]);
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_consts), sc, sizeof(sc)));
+    uint64_t recip[130];
+    for (uint32_t k = 0; k < 130; k++) recip[k] = lr_magic(k);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_lr_recip), recip, sizeof(recip)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), kGaussianSequence, sizeof(kGaussianSequence)));
     // LFSR step matrix (film_grain.rs:117-130) and its powers 2^b.
     uint16_t m[16][16];
@@ -504,7 +509,7 @@ static void sr_consts(uint32_t dw, uint32_t uw, uint32_t mi_cols, uint32_t sx, i
 // LR tile width for a unit size: must divide it; stripes never straddle units.
 // pref: 0 = the widest of 64/32 that divides the unit, else 32 or 64 when it divides.
 static int lr_tile_width(uint32_t us, uint32_t sy, uint32_t *tw, uint32_t pref = 0) {
-    if (us < 32 || us % 32 != 0 || us < (64u >> sy)) return OXAV1_ERR_UNSUPPORTED;
+    if (us < 32 || us % 32 != 0 || us < (64u >> sy) || us >= 65536u) return OXAV1_ERR_UNSUPPORTED;
     *tw = (us % 64 == 0) ? 64u : 32u;
     if (pref == 32) *tw = 32;
     return OXAV1_OK;
@@ -603,6 +608,19 @@ struct ScratchCarve {
 // demand) or, when carve is given, from the carve's region; then the
 // film-grain scratch header is returned in *fg_header for the caller to
 // store on the device instead of being copied here.
+// Reciprocals of the loop-restoration divisors (lr_mdiv): tile columns and
+// unit size per plane.  Item numbers within a plane and unit-row numerators
+// stay below 2^24 and the divisors below 2^16 for any frame AV1 can code.
+static int lr_set_magic(FrameJob &J) {
+    for (uint32_t p = 0; p < 3; p++) {
+        if (J.lr_tiles_x[p] >= 65536u || J.lr_unit_size[p] >= 65536u) return OXAV1_ERR_UNSUPPORTED;
+        J.lr_magic_tx[p] = lr_magic(J.lr_tiles_x[p]);
+        J.lr_magic_us[p] = lr_magic(J.lr_unit_size[p]);
+    }
+    if (J.item_base[kStageLr][3] >= (1u << 24)) return OXAV1_ERR_UNSUPPORTED;
+    return OXAV1_OK;
+}
+
 static int build_job(const oxav1_tuning &tun, const oxav1_frame_desc &d, const oxav1_frame_dev &v, FrameSlot &slot,
                      FrameJob &J, ScratchCarve *carve = nullptr, FgScratch *fg_header = nullptr) {
     std::memset(&J, 0, sizeof J);
@@ -816,6 +834,7 @@ static int build_job(const oxav1_tuning &tun, const oxav1_frame_desc &d, const o
         }
     }
     J.item_base[kStageLr][3] = acc;
+    if (int st = lr_set_magic(J)) return st;
     acc = 0;
     for (uint32_t p = 0; p < 3; p++) {
         J.item_base[kStageFg][p] = acc;

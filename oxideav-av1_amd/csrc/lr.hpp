@@ -24,9 +24,11 @@
 
 namespace oxav1 {
 
+constexpr int kLrWaves = 4;                 // waves per workgroup (256 threads)
 constexpr int kLrMaxW = 64;                 // max tile width (plane samples)
 constexpr int kLrMaxH = 32;                 // max rows of one chunk of a stripe
-constexpr int kLrSw = kLrMaxW + 6;          // source window width
+constexpr int kLrSw = kLrMaxW + 8;          // source window row pitch: window columns -1 .. W+6
+                                            // (the window's first column at LDS column 1)
 constexpr int kLrSh = kLrMaxH + 6;          // source window height
 constexpr int kLrAw = kLrMaxW + 2;          // A/B columns: tile cols -1 .. W
 constexpr int kLrAh1 = kLrMaxH + 2;         // pass-1 A/B rows: tile rows -1 .. H
@@ -37,8 +39,14 @@ constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // pass-0 A/B rows: the odd rows onl
 // windows need no rotating copies.
 constexpr int kLrBand = (kLrMaxH + 3) / 4;
 constexpr int kLrAbBand = (kLrMaxH + 2 + 2) / 3;
-// Source-window rows per wave with four waves (the window is kLrSh rows).
-constexpr int kLrWinRows = (kLrSh + 3) / 4;
+
+// n / d for n < 2^24 and 1 <= d < 2^16 as one multiply and shift: with
+// m = floor(2^40 / d) + 1 = 2^40 / d + e, 0 < e <= 1, n * m / 2^40 is
+// n / d + n * e / 2^40, and the second term is below 2^-16 < 1 / d, so the
+// floor is the quotient.
+__host__ __device__ inline uint64_t lr_magic(uint32_t d) { return (((uint64_t)1 << 40) / (d ? d : 1u)) + 1; }
+__device__ __forceinline__ uint32_t lr_mdiv(uint32_t n, uint64_t m) { return (uint32_t)(((uint64_t)n * m) >> 40); }
+__constant__ uint64_t c_lr_recip[130];      // lr_magic(k), k = 1..129
 
 // x * y for |x|, |y| < 2^23: the low 32 bits of the product (the same bits
 // a wrapping 32-bit multiply gives) on the full-rate 24-bit multiplier.
@@ -56,7 +64,7 @@ constexpr uint32_t lr_chunks(uint32_t sy) { return (64u >> sy) / kLrMaxH; }
 
 template <typename L>
 struct LrSmem {
-    L src[kLrSh * kLrSw];              // routed source window, rows ys-3 .. ye+3
+    alignas(16) L src[kLrSh * kLrSw];  // routed source window, rows ys-3 .. ye+3
     union {
         struct {
             uint32_t ab1[kLrAh1 * kLrAw];  // pass-1 (A, B)
@@ -64,8 +72,6 @@ struct LrSmem {
         };
         int32_t inter[kLrSh * kLrMaxW];    // Wiener horizontal pass
     };
-    uint64_t rowp[kLrSh];              // routed base address of each window row
-    int32_t rowmax[kLrSh];             // last column of that row's buffer; -1: none
     uint16_t a2[256];                  // c_sgr_a2 staged for lane-indexed reads
 };
 
@@ -126,22 +132,69 @@ __host__ __device__ inline LrSgrConsts lr_sgr_consts_of(int32_t r, int32_t e) {
     return c;
 }
 
-// Where a tile's restored samples go.  LrStore writes them to the output
-// plane; the encoder's election passes sinks that reduce them instead.
 // unit(): the tile's restoration unit (called once the unit is resolved);
 // put(): a copied or Wiener sample; sgr(): a self-guided sample with its
 // filter outputs (u = the CDEF sample, f0/f1 = the two passes' flt values,
 // u << 4 where a pass is off).
+// Four consecutive samples of T as one vector access.
+typedef uint32_t lr_u32x2 __attribute__((ext_vector_type(2)));
+typedef int32_t lr_i32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct LrVec;
+template <>
+struct LrVec<uint8_t> {
+    using type = uint32_t;
+};
+template <>
+struct LrVec<uint16_t> {
+    using type = lr_u32x2;
+};
+template <>
+struct LrVec<int32_t> {
+    using type = lr_i32x4;
+};
+// the four samples of v into LDS at d (8-byte aligned for u16, 16 for int32)
+template <typename T, typename L>
+__device__ __forceinline__ void lr_put4(L *d, typename LrVec<T>::type v) {
+    if constexpr (std::is_same<T, uint8_t>::value) {
+        *(lr_u32x2 *)d = lr_u32x2{(v & 0xffu) | ((v << 8) & 0xff0000u), ((v >> 16) & 0xffu) | ((v >> 8) & 0xff0000u)};
+    } else {
+        *(typename LrVec<T>::type *)d = v;
+    }
+}
+
+// Where a tile's restored samples go.  LrStore writes them to the output
+// plane; the encoder's election passes sinks that reduce them instead.
+// kCopy4: the sink takes a NONE tile's copy four samples at a time (copy4).
 template <typename T>
 struct LrStore {
+    static constexpr bool kCopy4 = true;
     G<T> *__restrict__ out;
     uint32_t opitch;
+    __device__ __forceinline__ bool copy4_ok() const {
+        return (opitch & 3u) == 0 && ((uintptr_t)out & (4 * sizeof(T) - 1)) == 0;
+    }
+    __device__ __forceinline__ void copy4(int32_t y, int32_t x, typename LrVec<T>::type v) {
+        *(G<typename LrVec<T>::type> *)(out + (size_t)y * opitch + x) = v;
+    }
     __device__ __forceinline__ void unit(uint32_t, uint32_t, const oxav1_lr_unit &) {}
     __device__ __forceinline__ void put(int32_t y, int32_t x, int32_t v) { out[(size_t)y * opitch + x] = (T)v; }
     __device__ __forceinline__ void sgr(int32_t y, int32_t x, int32_t v, int32_t, int32_t, int32_t) {
         out[(size_t)y * opitch + x] = (T)v;
     }
 };
+
+// (stripe, chunk, column tile) of item t of plane p's loop-restoration items,
+// numbered stripe-major, then chunk, then tile.
+template <class JT>
+__device__ __forceinline__ void lr_decode(JT &J, uint32_t p, uint32_t t, uint32_t &stripe, uint32_t &chunk,
+                                          uint32_t &tile) {
+    const uint32_t q = lr_mdiv(t, J.lr_magic_tx[p]);  // stripe * chunks + chunk
+    tile = t - q * J.lr_tiles_x[p];
+    const bool two = lr_chunks(p ? J.sy : 0) == 2;
+    stripe = two ? q >> 1 : q;
+    chunk = two ? q & 1u : 0u;
+}
 
 // One tile: (plane, stripe, column tile) -> sink (cropped frame coordinates
 // equal plane coordinates).  Emits rows [ys, ye] x cols [xs, xe].  JT is the
@@ -169,7 +222,8 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     const int tid = threadIdx.x, nt = blockDim.x;
     // the wave index is uniform in a wave: a scalar, so row bands and their
     // base addresses are scalar too
-    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), nwv = nt >> 6;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int nwv = kLrWaves;
     const int32_t W = g.xe - g.xs + 1;
     const G<T> *cdf = plane_ptr<T>(J.up_cdef[plane]);
     const uint32_t dpitch = J.up_cdef_pitch[plane];
@@ -184,9 +238,9 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     uint32_t type = J.uses_lr ? (types >> (8 * plane)) & 0xffu : OXAV1_RESTORE_NONE;
     oxav1_lr_unit u{};
     if (type != OXAV1_RESTORE_NONE) {
-        const uint32_t us = J.lr_unit_size[plane];
-        const uint32_t ur = min(J.lr_unit_rows[plane] - 1, (uint32_t)(g.ys + (8 >> sy)) / us);
-        const uint32_t uc = min(J.lr_unit_cols[plane] - 1, (uint32_t)g.xs / us);
+        const uint64_t mus = J.lr_magic_us[plane];
+        const uint32_t ur = min(J.lr_unit_rows[plane] - 1, lr_mdiv((uint32_t)(g.ys + (8 >> sy)), mus));
+        const uint32_t uc = min(J.lr_unit_cols[plane] - 1, lr_mdiv((uint32_t)g.xs, mus));
         u = J.lr_units[plane] ? J.lr_units[plane][(size_t)ur * J.lr_unit_cols[plane] + uc] : oxav1_lr_unit{};
         type = u.type;
         sink.unit(ur, uc, u);
@@ -197,10 +251,29 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     if (g.ys > g.ye) return;
     const int32_t H = g.ye - g.ys + 1;
     // Output rows are split into one band per wave; a lane owns a column.
-    const int RB = (H + nwv - 1) / nwv;
+    const int RB = (H + nwv - 1) >> 2;
     const int b0 = wv * RB, b1 = min(H, b0 + RB);
+    using V = typename LrVec<T>::type;
     if (type != OXAV1_RESTORE_WIENER && type != OXAV1_RESTORE_SGRPROJ) {
         if (!J.lr_copy_none) return;
+        if constexpr (Sink::kCopy4) {
+          if (sink.copy4_ok() && (dpitch & 3u) == 0 && ((uintptr_t)cdf & (4 * sizeof(T) - 1)) == 0) {
+            // four samples per lane (the tile starts at a multiple of 32)
+            const int gw = (W + 3) >> 2;
+            const uint64_t mg = c_lr_recip[gw];
+            for (int i = tid; i < H * gw; i += nt) {
+                const int r = (int)lr_mdiv((uint32_t)i, mg), gi = i - r * gw;
+                const int32_t y = g.ys + r, x = g.xs + 4 * gi;
+                const G<T> *q = cdf + (size_t)y * dpitch + x;
+                if (x + 3 <= g.xe) {
+                    sink.copy4(y, x, *(const G<V> *)q);
+                } else {
+                    for (int k = 0; k < 4 && x + k <= g.xe; k++) sink.put(y, x + k, (int32_t)q[k]);
+                }
+            }
+            return;
+          }
+        }
         // every load of the band is issued before the first store, so the
         // copy waits for memory once, not once per row
         for (int x = lane; x < W; x += 64)
@@ -217,69 +290,71 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     }
     // Routed source window (get_source_sample, loop_restoration.rs:862-902):
     // rows above/below the stripe come from the pre-CDEF frame (at most 2
-    // rows out), the rest from the CDEF frame; then both clamps.
+    // rows out), the rest from the CDEF frame; then both clamps.  The window
+    // (rows ys-3 .. ye+3, columns xs-3 .. xe+3) is loaded in groups of four
+    // columns starting at xs-4 (a multiple of four): window column c sits at
+    // LDS column c + 1, so every group is one aligned LDS store.  A lane
+    // routes its group's row and loads the four samples with one vector
+    // load where they lie inside the row's buffer, clamped per sample at the
+    // plane's edges.  All of a lane's groups are loaded before the first
+    // store (one memory round trip); rows without a buffer store 0.
     g.wy0 = g.ys - 3;
     g.wx0 = g.xs - 3;
     g.wh = H + 6;
     g.ww = W + 6;
-    if (tid < g.wh) {
-        const int32_t crows = (int32_t)J.lrc_rows[plane], ccols = (int32_t)J.lrc_cols[plane];
-        const int32_t rows = (int32_t)J.lr_rows[plane], cols = (int32_t)J.lr_cols[plane];
-        int32_t y = clip3(0, max(g.pey, 0), g.wy0 + tid);
-        const G<T> *b = cdf;
-        uint32_t pitch = dpitch;
-        int32_t nr = rows, nc = cols;
-        if (y < g.ss || y > g.se) {
-            y = y < g.ss ? max(y, g.ss - 2) : min(y, g.se + 2);
-            b = plane_ptr<T>(J.up_deb[plane]);
-            pitch = J.up_deb_pitch[plane];
-            nr = crows;
-            nc = ccols;
-        }
-        if (nr == 0 || nc == 0) {
-            S.rowp[tid] = 0;
-            S.rowmax[tid] = -1;
-        } else {
-            y = clip3(0, nr - 1, y);
-            S.rowp[tid] = (uint64_t)(b + (size_t)y * pitch);
-            S.rowmax[tid] = nc - 1;
-        }
-    }
     if (type == OXAV1_RESTORE_SGRPROJ)
         for (int i = tid; i < 256; i += nt) S.a2[i] = c_sgr_a2[i];
-    __syncthreads();
     {
-        // Window rows go to the waves in turn, lanes to columns 0..63 and
-        // 64..ww-1, so the column clamps are computed once per lane.  All of
-        // a wave's rows (at most kLrWinRows) are loaded before the first
-        // store, so the window costs one memory round trip; every load is in
-        // bounds: rows past the window repeat the last row (no store), rows
-        // without a buffer read the CDEF plane's first sample and store 0.
+        const int32_t crows = (int32_t)J.lrc_rows[plane], ccols = (int32_t)J.lrc_cols[plane];
+        const int32_t rows = (int32_t)J.lr_rows[plane], cols = (int32_t)J.lr_cols[plane];
+        const G<T> *deb = plane_ptr<T>(J.up_deb[plane]);
+        const uint32_t bpitch = J.up_deb_pitch[plane];
+        const bool al_c = (dpitch & 3u) == 0 && ((uintptr_t)cdf & (4 * sizeof(T) - 1)) == 0;
+        const bool al_d = (bpitch & 3u) == 0 && ((uintptr_t)deb & (4 * sizeof(T) - 1)) == 0;
         const int32_t xmax = max(g.pex, 0);
-        const int32_t xa = clip3(0, xmax, g.wx0 + lane);
-        const int32_t xb = clip3(0, xmax, g.wx0 + lane + 64);
-        const bool st_a = lane < g.ww, st_b = lane + 64 < g.ww;
-        const int nrw = (g.wh - wv + nwv - 1) / nwv;  // this wave's rows (uniform in the wave)
-        for (int i0 = 0; i0 < nrw; i0 += kLrWinRows) {
-            L va[kLrWinRows], vb[kLrWinRows];
+        const int ngx = (g.ww + 1 + 3) >> 2;  // groups per row: window columns -1 .. ww-1
+        const uint64_t mg = c_lr_recip[ngx];
+        const int ng = g.wh * ngx;
+        constexpr int kG = (kLrSh * (kLrSw / 4) + 255) / 256;  // groups per lane
+        V v[kG];
+        int off[kG];
 #pragma unroll
-            for (int j = 0; j < kLrWinRows; j++) {
-                const int rr = min(wv + (i0 + j) * nwv, g.wh - 1);
-                const int32_t cm = S.rowmax[rr];
-                const G<T> *row = cm < 0 ? cdf : (const G<T> *)S.rowp[rr];
-                const int32_t c = max(cm, 0);
-                va[j] = cm < 0 ? (L)0 : (L)row[min(xa, c)];
-                vb[j] = cm < 0 ? (L)0 : (L)row[min(xb, c)];
-            }
+        for (int j = 0; j < kG; j++) {
+            const int i = tid + j * 256;
+            off[j] = -1;
+            if (i < ng) {
+                const int r = (int)lr_mdiv((uint32_t)i, mg), gi = i - r * ngx;
+                int32_t y = clip3(0, max(g.pey, 0), g.wy0 + r);
+                const bool alt = y < g.ss || y > g.se;
+                if (alt) y = y < g.ss ? max(y, g.ss - 2) : min(y, g.se + 2);
+                const int32_t nr = alt ? crows : rows, nc = alt ? ccols : cols;
+                const G<T> *b = alt ? deb : cdf;
+                const uint32_t pitch = alt ? bpitch : dpitch;
+                off[j] = r * kLrSw + 4 * gi;
+                if (nr <= 0 || nc <= 0) {
+                    v[j] = V{};
+                } else {
+                    const G<T> *row = b + (size_t)clip3(0, nr - 1, y) * pitch;
+                    const int32_t x0 = g.xs - 4 + 4 * gi, cm = min(xmax, nc - 1);
+                    if (x0 >= 0 && x0 + 3 <= cm && (alt ? al_d : al_c)) {
+                        v[j] = *(const G<V> *)(row + x0);
+                    } else {
+                        T t[4];
 #pragma unroll
-            for (int j = 0; j < kLrWinRows; j++) {
-                const int r = wv + (i0 + j) * nwv;
-                if (i0 + j < nrw) {
-                    if (st_a) S.src[r * kLrSw + lane] = va[j];
-                    if (st_b) S.src[r * kLrSw + lane + 64] = vb[j];
+                        for (int k = 0; k < 4; k++) t[k] = row[min(clip3(0, xmax, x0 + k), nc - 1)];
+                        if constexpr (std::is_same<T, uint8_t>::value)
+                            v[j] = (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16 | (uint32_t)t[3] << 24;
+                        else if constexpr (std::is_same<T, uint16_t>::value)
+                            v[j] = V{(uint32_t)t[0] | (uint32_t)t[1] << 16, (uint32_t)t[2] | (uint32_t)t[3] << 16};
+                        else
+                            v[j] = V{t[0], t[1], t[2], t[3]};
+                    }
                 }
             }
         }
+#pragma unroll
+        for (int j = 0; j < kG; j++)
+            if (off[j] >= 0) lr_put4<T>(S.src + off[j], v[j]);
     }
     __syncthreads();
     const int32_t bd = J.bd;
@@ -304,7 +379,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
         for (int rb = 0; rb < H + 6; rb += nwv) {
             const int wr = rb + wv;
             if (wr < H + 6 && lane < W) {
-                const L *q = S.src + wr * kLrSw + lane;
+                const L *q = S.src + 1 + wr * kLrSw + lane;
                 // u16 samples: |taps| < 2^9 and sample sums < 2^17, 24-bit multiplies
                 constexpr bool k24h = std::is_same<L, uint16_t>::value;
                 auto hm = [](int32_t x, int32_t y) -> int32_t { return k24h ? lr_mul24(x, y) : x * y; };
@@ -348,9 +423,10 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     // (loop_restoration.rs:686-699).
     {
         const int ncol = W + 2;
-        const int nb = nt / ncol;
-        const int c = tid % ncol, band = tid / ncol;
-        const int KR = (H + 2 + nb - 1) / nb;
+        const uint64_t mc = c_lr_recip[ncol];
+        const int nb = (int)lr_mdiv((uint32_t)nt, mc);
+        const int band = (int)lr_mdiv((uint32_t)tid, mc), c = tid - band * ncol;
+        const int KR = (int)lr_mdiv((uint32_t)(H + 2 + nb - 1), c_lr_recip[nb]);
         const int k0 = -1 + band * KR, k1 = min(H, k0 + KR - 1);  // inclusive tile rows
         if (band < nb && k0 <= k1) {
             const LrSgrConsts kc0 = c_sgr_consts[set][0], kc1 = c_sgr_consts[set][1];
@@ -359,7 +435,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
             // window row of tile row k is k + 3; column of tile col c - 1 is c + 2
             int32_t h3[5], q3[5], h5[5], q5[5];
             auto hsum = [&](int wr, int slot) {
-                const L *q = S.src + wr * kLrSw + c;  // cols c .. c+4 = tile cols c-3 .. c+1
+                const L *q = S.src + 1 + wr * kLrSw + c;  // cols c .. c+4 = tile cols c-3 .. c+1
                 const int32_t v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3], v4 = q[4];
                 h3[slot] = v1 + v2 + v3;
                 q3[slot] = v1 * v1 + v2 * v2 + v3 * v3;
@@ -450,7 +526,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
         for (int i = 0; i < kLrBand; i++) {
             const int k = kb + i;
             if (k >= b1) break;
-            const int32_t uval = (int32_t)S.src[(k + 3) * kLrSw + lane + 3];
+            const int32_t uval = (int32_t)S.src[1 + (k + 3) * kLrSw + lane + 3];
             const int32_t u4 = uval << 4;
             int32_t f0 = u4, f1 = u4;
             if (r0) {
