@@ -140,7 +140,11 @@ __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8
     __syncthreads();
     const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
     const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
-    fg_blend_tile<T>(J, p, y0, y1, x0, x1, lut);
+    // the job record is read-only for the whole launch: through the constant
+    // address space its fields are scalar loads (fg_blend_tile indexes no
+    // byte array of it by the plane)
+    using JC = const __attribute__((address_space(4))) FrameJob;
+    fg_blend_tile<T, JC>(*(JC *)&J, p, y0, y1, x0, x1, lut);
 }
 
 // --- kernels: KIND 0 = engine (u8/u16 by FrameJob::kind), 1 = int32 ---

@@ -21,6 +21,7 @@
 namespace oxav1 {
 
 constexpr int kGrainW = 82, kGrainH = 73;
+constexpr int kTqW = 84;  // row pitch of the shifted template copies (a multiple of four)
 
 struct FgScratch {
     int16_t luma[kGrainH * kGrainW];
@@ -30,6 +31,10 @@ struct FgScratch {
     int32_t chroma_w, chroma_h;
     uint32_t nstripes, nblocks;
     uint8_t *offsets;  // nstripes * nblocks LFSR draws
+    // tq[p][k]: plane p's template shifted left by k columns (row pitch
+    // kTqW, zero past the template), so the four values at any column c of
+    // a row are one aligned 8-byte read from copy c & 3 at column c & ~3
+    alignas(16) int16_t tq[3][4][kGrainH * kTqW];
 };
 
 __constant__ int16_t c_gauss[2048];       // filled from kGaussianSequence at init
@@ -295,6 +300,19 @@ __device__ void fg_prep_frame(const FrameJob &J, int16_t *sm /* 4 * 73 * 82 */) 
         S->cb[i] = Cb[i];
         S->cr[i] = Cr[i];
     }
+    // the shifted copies, a (plane, shift, row) per kTqW threads
+    constexpr int kRowsPerPass = 256 / kTqW;
+    const int c = tid % kTqW, rr = tid / kTqW;
+    for (int row0 = 0; row0 < 3 * 4 * kGrainH; row0 += kRowsPerPass) {
+        const int row = row0 + rr;
+        if (rr >= kRowsPerPass || row >= 3 * 4 * kGrainH) continue;
+        const int p = row / (4 * kGrainH), k = (row / kGrainH) & 3, r = row % kGrainH;
+        const int16_t *t = p == 0 ? L : (p == 1 ? Cb : Cr);
+        const int tw = p == 0 ? kGrainW : cw, th = p == 0 ? kGrainH : ch;
+        int16_t v = 0;
+        if (r < th && c + k < tw) v = t[r * tw + c + k];
+        ((G<int16_t> *)S->tq[p][k])[r * kTqW + c] = v;
+    }
 }
 
 // Grain sample of (stripe s, block b, row i, col j) for a plane.
@@ -362,18 +380,115 @@ __device__ __forceinline__ int32_t fg_scale(const uint8_t *t, int32_t index, uin
     return a + fg_round2((b - a) * rem, shift);
 }
 
+// Four consecutive samples of T as one vector (raw, packed as loaded).
+typedef uint32_t fg_u32x2 __attribute__((ext_vector_type(2)));
+typedef int32_t fg_i32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct FgV;
+template <>
+struct FgV<uint8_t> {
+    using type = uint32_t;
+};
+template <>
+struct FgV<uint16_t> {
+    using type = fg_u32x2;
+};
+template <>
+struct FgV<int32_t> {
+    using type = fg_i32x4;
+};
+// sample k of a raw vector
+template <typename T>
+__device__ __forceinline__ int32_t fg_at(const typename FgV<T>::type &v, int k) {
+    if constexpr (sizeof(T) == 1)
+        return (int32_t)((v >> (8 * k)) & 255u);
+    else if constexpr (sizeof(T) == 2)
+        return (int32_t)(((k < 2 ? v.x : v.y) >> (16 * (k & 1))) & 0xffffu);
+    else
+        return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+// four samples packed into a raw vector
+template <typename T>
+__device__ __forceinline__ typename FgV<T>::type fg_pack(const int32_t (&t)[4]) {
+    if constexpr (sizeof(T) == 1)
+        return (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16 | (uint32_t)t[3] << 24;
+    else if constexpr (sizeof(T) == 2)
+        return fg_u32x2{(uint32_t)t[0] | (uint32_t)t[1] << 16, (uint32_t)t[2] | (uint32_t)t[3] << 16};
+    else
+        return fg_i32x4{t[0], t[1], t[2], t[3]};
+}
+// Four samples of a plane row at p: one vector load where vec (p aligned to
+// four samples, all four columns inside), else the first n one by one.
+template <typename T>
+__device__ __forceinline__ typename FgV<T>::type fg_ld4(const G<T> *p, bool vec, int n) {
+    if (vec) return *(const G<typename FgV<T>::type> *)p;
+    int32_t t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) t[k] = k < n ? (int32_t)p[k] : 0;
+    return fg_pack<T>(t);
+}
+template <typename T>
+__device__ __forceinline__ void fg_st4(G<T> *p, bool vec, int n, const int32_t (&v)[4]) {
+    if (vec) {
+        *(G<typename FgV<T>::type> *)p = fg_pack<T>(v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < n) p[k] = (T)v[k];
+    }
+}
+// four consecutive template values at (row r, column c) of plane p's
+// template, from the shifted copy c & 3
+__device__ __forceinline__ fg_u32x2 fg_tq4(const G<int16_t> *tq, int32_t r, int32_t c) {
+    return *(const G<fg_u32x2> *)(tq + ((uint32_t)(c & 3) * (kGrainH * kTqW) + (uint32_t)(r * kTqW + (c & ~3))));
+}
+__device__ __forceinline__ int32_t fg_tq_at(const fg_u32x2 &v, int k) {
+    const uint32_t w = k < 2 ? v.x : v.y;
+    return (k & 1) ? (int32_t)w >> 16 : (int32_t)(int16_t)(w & 0xffffu);
+}
+
+// Clamp of v to [lo, hi] (lo <= hi) as one v_med3_i32.
+__device__ __forceinline__ int32_t fg_med3(int32_t v, int32_t lo, int32_t hi) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+    return r;
+}
+// a * b: u8/u16 frames keep every product of the blend within the 24-bit
+// multiplier's operand range (samples < 2^12, noise and weights < 2^12);
+// the int32 drop-in path keeps the reference's 32-bit products.
+template <typename T>
+__device__ __forceinline__ int32_t fg_mul(int32_t a, int32_t b) {
+    if constexpr (sizeof(T) == 4)
+        return a * b;
+    else
+        return __mul24(a, b);
+}
+// scale_lut (film_grain.rs:543-555) without a branch: the top entry (or any
+// index whose bucket is 255 or more) has its next entry equal to itself, so
+// the interpolation adds nothing there.  rnd = Round2's constant of shift.
+template <typename T>
+__device__ __forceinline__ int32_t fg_scale_at(const uint8_t *t, int32_t index, uint32_t shift, int32_t rnd) {
+    const uint32_t x = min((uint32_t)(index >> shift), 255u);
+    const int32_t a = t[x], b = t[min(x + 1, 255u)];
+    const int32_t rem = index - (int32_t)(x << shift);
+    return a + ((fg_mul<T>(b - a, rem) + rnd) >> shift);
+}
+
 // Blend one tile (rows [y0, y1), cols [x0, x1), at most 64 x 64, tile
 // corners on multiples of 64) of one plane: in (pre-grain) -> out
-// (add_noise_synthesis, film_grain.rs:570-829).  A lane owns a column and
-// each wave a band of 16 rows.  Bands never straddle a noise stripe (32 or
-// 16 rows) and lanes never straddle a noise block, so the block offsets and
-// template row pointers are resolved once per lane, and the overlap blends
-// only touch the first columns of a block and the first rows of a stripe.
-template <typename T>
-__device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
+// (add_noise_synthesis, film_grain.rs:570-829).  A lane owns four
+// consecutive columns of one row (16 lanes per row); each wave walks a band
+// of 16 rows, four rows at a time, with every load of the band issued
+// before the first store.  Bands never straddle a noise stripe (32 or 16
+// rows) and a lane's columns never straddle a noise block (32 or 16
+// columns), so the block offsets are resolved once per lane, the horizontal
+// overlap blend only touches the lanes at a block's first columns and the
+// vertical one the first rows of a stripe.
+template <typename T, class JT = const FrameJob>
+__device__ __forceinline__ void fg_blend_tile(JT &J, uint32_t plane, int32_t y0, int32_t y1, int32_t x0, int32_t x1,
                               const uint8_t *lut /* LDS [3][256] */) {
     const G<FgScratch> *S = (const G<FgScratch> *)J.fgs;
-    const oxav1_film_grain_params &fg = J.fg;
+    auto &fg = J.fg;
     const uint32_t bd = J.bd;
     const uint32_t psx = plane ? J.sx : 0, psy = plane ? J.sy : 0;
     const int32_t gc = 128 << (bd - 8);
@@ -388,17 +503,23 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
         maxl = maxc = (256 << (bd - 8)) - 1;
     }
     const uint32_t gs = fg.grain_scaling;
+    const int32_t rndg = gs ? 1 << (gs - 1) : 0;              // Round2 constants
+    const uint32_t shift = bd - 8;
+    const int32_t rnds = shift ? 1 << (shift - 1) : 0;
     const bool overlap = fg.overlap_flag != 0;
     const G<T> *in = plane_ptr<T>(J.ref_out[plane]);
     G<T> *out = plane_ptr<T>(J.out[plane]);
     const uint32_t ip = J.ref_pitch[plane], op = J.out_pitch[plane];
     const int32_t w = (int32_t)J.out_cols[0];
-    // the wave index is uniform in a wave: a scalar, so the band's rows and
-    // their base addresses are scalar too
+    // the wave index is uniform in a wave: a scalar, so the band's geometry is too
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int32_t x = x0 + lane;
+    const int rs = lane >> 4;                         // row within a chunk of four
+    const int32_t x = x0 + 4 * (lane & 15);
     const int32_t yb0 = y0 + 16 * wv, yb1 = min(y1, yb0 + 16);
-    if (x >= x1 || yb0 >= yb1) return;
+    if (yb0 >= yb1) return;
+    const int n = min(4, x1 - x);                     // this lane's columns (<= 0: none)
+    const bool vin = (ip & 3u) == 0 && ((uintptr_t)in & (4 * sizeof(T) - 1)) == 0 && n == 4;
+    const bool vout = (op & 3u) == 0 && ((uintptr_t)out & (4 * sizeof(T) - 1)) == 0 && n == 4;
     // A job without grain (apply_grain == 0, out != ref_out) has no grain
     // scratch: it only copies.
     bool active;
@@ -412,122 +533,189 @@ __device__ __forceinline__ void fg_blend_tile(const FrameJob &J, uint32_t plane,
                  (plane == 1 ? (fg.num_cb_points > 0 || fg.chroma_scaling_from_luma)
                              : (fg.num_cr_points > 0 || fg.chroma_scaling_from_luma));
     }
-    const G<T> *src = in + (size_t)yb0 * ip + x;
-    G<T> *dst = out + (size_t)yb0 * op + x;
+    using V = typename FgV<T>::type;
     if (!active) {
-        for (int32_t y = yb0; y < yb1; y++, src += ip, dst += op) *dst = *src;
+        V v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int32_t y = yb0 + 4 * c + rs;
+            if (n > 0 && y < yb1) v[c] = fg_ld4<T>(in + ((uint32_t)y * ip + (uint32_t)x), vin, n);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int32_t y = yb0 + 4 * c + rs;
+            if (n > 0 && y < yb1) {
+                int32_t t[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) t[k] = fg_at<T>(v[c], k);
+                fg_st4<T>(out + ((uint32_t)y * op + (uint32_t)x), vout, n, t);
+            }
+        }
         return;
     }
     const G<int16_t> *tmpl = plane == 0 ? S->luma : (plane == 1 ? S->cb : S->cr);
+    const G<int16_t> *tq = S->tq[plane][0];
     const int tw = plane == 0 ? kGrainW : (int)S->chroma_w;
     // noise block / stripe geometry (film_grain.rs:626-651, 705-739)
     const uint32_t bsh = 5 - psx, ssh = 5 - psy;       // log2 block width / stripe height
-    const uint32_t b = (uint32_t)x >> bsh;
-    const int32_t j = x & ((1 << bsh) - 1);
+    const uint32_t b = (uint32_t)max(x, 0) >> bsh;
+    const int32_t j = x & ((1 << bsh) - 1);            // a multiple of four
     const uint32_t st = (uint32_t)yb0 >> ssh;
-    const int32_t ib = yb0 - (int32_t)(st << ssh);       // row of yb0 inside its stripe
+    const int32_t ib = yb0 - (int32_t)(st << ssh);       // row of yb0 inside its stripe (uniform)
     const G<uint8_t> *offs = (const G<uint8_t> *)S->offsets;
     const uint32_t nblocks = S->nblocks;
-    auto block_base = [&](uint32_t ss, uint32_t bb) -> const G<int16_t> * {
+    // (column, row) of a block's noise in the template
+    auto block_pos = [&](uint32_t ss, uint32_t bb, int32_t &pox, int32_t &poy) {
         const uint32_t rnd = offs[ss * nblocks + bb];
         const int32_t ox = (int32_t)(rnd >> 4), oy = (int32_t)(rnd & 15u);
-        const int32_t pox = psx ? 6 + ox : 9 + 2 * ox;
-        const int32_t poy = psy ? 6 + oy : 9 + 2 * oy;
-        return tmpl + poy * tw + pox;
+        pox = psx ? 6 + ox : 9 + 2 * ox;
+        poy = psy ? 6 + oy : 9 + 2 * oy;
     };
     // horizontal overlap: the first 2 (1 when subsampled) columns of a block
     // blend with the previous block's columns 32, 33 (16)
-    const bool hov = overlap && b > 0 && j < (psx ? 1 : 2);
-    const int32_t hw_old = psx ? 23 : (j == 0 ? 27 : 17), hw_new = psx ? 22 : (j == 0 ? 17 : 27);
-    const G<int16_t> *pc = block_base(st, b) + j;
-    const G<int16_t> *pl = hov ? block_base(st, b - 1) + j + (1 << bsh) : pc;
+    const int nh = psx ? 1 : 2;
+    const bool hov = overlap && b > 0 && j == 0;
+    int32_t pcx = 0, pcy = 0, plx = 0, ply = 0;
+    if (n > 0) block_pos(st, b, pcx, pcy);
+    if (n > 0 && hov) block_pos(st, b - 1, plx, ply);
+    plx += 1 << bsh;
     // vertical overlap: the first 2 (1) rows of a stripe blend with rows 32,
-    // 33 (16) of the previous stripe's noise
-    const int32_t vrows = (overlap && st > 0) ? (psy ? 1 : 2) : 0;
-    const G<int16_t> *pu = pc, *pul = pl;
-    if (vrows > ib) {
-        pu = block_base(st - 1, b) + j;
-        pul = hov ? block_base(st - 1, b - 1) + j + (1 << bsh) : pu;
+    // 33 (16) of the previous stripe's noise; a band holds them only when it
+    // starts the stripe (ib == 0)
+    const int32_t vrows = (overlap && st > 0 && ib == 0) ? (psy ? 1 : 2) : 0;
+    int32_t pux = 0, puy = 0, pulx = 0, puly = 0;
+    if (vrows > 0 && n > 0) {
+        block_pos(st - 1, b, pux, puy);
+        if (hov) block_pos(st - 1, b - 1, pulx, puly);
+        puy += 1 << ssh;
+        puly += 1 << ssh;
+        pulx += 1 << bsh;
     }
     const uint8_t *lt = lut + 256 * plane;
     const G<T> *ly = plane_ptr<T>(J.ref_out[0]);
     const uint32_t lp = J.ref_pitch[0];
-    const int32_t lx = x << psx, lnx = min(lx + 1, w - 1);
     const bool from_luma = fg.chroma_scaling_from_luma != 0;
     const int32_t lm = plane == 1 ? fg.cb_luma_mult : fg.cr_luma_mult;
     const int32_t mm = plane == 1 ? fg.cb_mult : fg.cr_mult;
     const int32_t coff = ((plane == 1 ? fg.cb_offset : fg.cr_offset) - 256) * (1 << (bd - 8));
     const int32_t maxs = (1 << bd) - 1;
-    // The band is blended kFgRows rows at a time: every load of a chunk
-    // (template values, source samples, co-located luma) is issued before
-    // the first store, at rows clamped to the band, so a chunk costs one
-    // memory round trip instead of one per row.  The vertical-overlap rows
-    // (the first 2, or 1 when vertically subsampled, rows of a noise stripe)
-    // only start a band, so their previous-stripe values are loaded with
-    // every chunk (used by the first); every such load stays inside the
-    // template: block bases are at most 39 rows down in a 73-row template
-    // (21 in a 38-row one when subsampled), and the rows read are 32, 33
-    // (16) below them.
-    constexpr int kFgRows = 4;
-    const int32_t ovr = 1 << ssh;  // rows of a stripe's noise
-    // 32-bit element offsets from the (wave-uniform) plane and template
-    // bases, so each load needs one address register
-    const int32_t oc_ = (int32_t)(pc - tmpl), ol_ = (int32_t)(pl - tmpl);
-    const int32_t ou_ = (int32_t)(pu - tmpl), oul_ = (int32_t)(pul - tmpl);
-    const uint32_t so_ = (uint32_t)yb0 * ip + (uint32_t)x, do_ = (uint32_t)yb0 * op + (uint32_t)x;
-    for (int32_t yc = yb0; yc < yb1; yc += kFgRows) {
-        const int32_t n = min(kFgRows, yb1 - yc);
-        int32_t nzc[kFgRows], nzl[kFgRows], org[kFgRows], la[kFgRows], lb[kFgRows];
+    // co-located luma of the lane's four columns: columns x << psx .. ,
+    // the odd one of each pair clamped to the luma width
+    const int32_t lx = x << psx;
+    const bool vl = (lp & 3u) == 0 && ((uintptr_t)ly & (4 * sizeof(T) - 1)) == 0 && lx + (4 << psx) <= w;
+    // The band is blended a chunk of four rows at a time: every load of a
+    // chunk is issued before its first store and kept as loaded.
+    const int32_t hwo0 = psx ? 23 : 27, hwn0 = psx ? 22 : 17;  // column 0 of a block (row 0 of a stripe)
+    fg_u32x2 vor = {0u, 0u};  // the previous stripe's overlap row (first chunk, rows < vrows)
+    int32_t vl2[2] = {0, 0};
+    if (vrows > 0 && n > 0 && rs < vrows) {
+        vor = fg_tq4(tq, puy + rs, pux + j);
+        if (hov)
+            for (int k = 0; k < nh; k++) vl2[k] = tmpl[(puly + rs) * tw + pulx + k];
+    }
+    constexpr int kCh = 1;  // chunks per step
+#pragma unroll 1
+    for (int h = 0; h < 4; h += kCh) {
+        fg_u32x2 nzr[kCh];
+        V org[kCh], lar[kCh], lbr[kCh];
+        int32_t hl[kCh][2] = {};
 #pragma unroll
-        for (int r = 0; r < kFgRows; r++) {
-            const int32_t dy = yc - yb0 + min(r, n - 1);
-            const int32_t i = ib + dy;
-            nzc[r] = tmpl[oc_ + i * tw];
-            nzl[r] = tmpl[ol_ + i * tw];
-            org[r] = (int32_t)in[so_ + (uint32_t)dy * ip];
-            if (plane) {
-                const uint32_t lo = (uint32_t)((yb0 + dy) << psy) * lp;
-                la[r] = (int32_t)ly[lo + (uint32_t)lx];
-                lb[r] = (int32_t)ly[lo + (uint32_t)lnx];
+        for (int cc = 0; cc < kCh; cc++) {
+            const int c = h + cc;
+            const int32_t y = yb0 + 4 * c + rs;
+            const bool ok = n > 0 && y < yb1;
+            const int32_t i = ib + 4 * c + rs;
+            if (ok) {
+                nzr[cc] = fg_tq4(tq, pcy + i, pcx + j);
+                org[cc] = fg_ld4<T>(in + ((uint32_t)y * ip + (uint32_t)x), vin, n);
+                if (hov)
+                    for (int k = 0; k < nh; k++) hl[cc][k] = tmpl[(ply + i) * tw + plx + k];
+            }
+            if (plane && ok) {
+                const G<T> *lrow = ly + (uint32_t)(y << psy) * lp;
+                if (vl) {
+                    lar[cc] = fg_ld4<T>(lrow + lx, true, 4);
+                    if (psx) lbr[cc] = fg_ld4<T>(lrow + lx + 4, true, 4);
+                } else {
+                    // at the right edge: luma columns clamped to the luma width
+                    int32_t t0[4], t1[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        t0[k] = (int32_t)lrow[min(lx + k, w - 1)];
+                        t1[k] = psx ? (int32_t)lrow[min(lx + 4 + k, w - 1)] : 0;
+                    }
+                    lar[cc] = fg_pack<T>(t0);
+                    lbr[cc] = fg_pack<T>(t1);
+                }
             }
         }
-        int32_t oc[2], ol[2];
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int32_t row = (psy ? 0 : r) + ovr;
-            oc[r] = tmpl[ou_ + row * tw];
-            ol[r] = tmpl[oul_ + row * tw];
-        }
+        for (int cc = 0; cc < kCh; cc++) {
+            const int c = h + cc;
+            const int32_t y = yb0 + 4 * c + rs;
+            if (n <= 0 || y >= yb1) continue;
+            const int32_t i = ib + 4 * c + rs;
+            // the chunk's noise with the overlap blends: (27, 17) at column /
+            // row 0, (17, 27) at 1; (23, 22) when subsampled
+            int32_t v[4];
 #pragma unroll
-        for (int r = 0; r < kFgRows; r++) {
-            if (r >= n) break;
-            const int32_t dy = yc - yb0 + r, i = ib + dy;
-            int32_t nz = nzc[r];
-            if (hov) nz = clip3(gmin, gmax, fg_round2(nzl[r] * hw_old + nz * hw_new, 5));
-            if (i < vrows) {
-                int32_t old = oc[i];
-                if (hov) old = clip3(gmin, gmax, fg_round2(ol[i] * hw_old + old * hw_new, 5));
-                const int32_t wo = psy ? 23 : (i == 0 ? 27 : 17), wn = psy ? 22 : (i == 0 ? 17 : 27);
-                nz = clip3(gmin, gmax, fg_round2(old * wo + nz * wn, 5));
+            for (int k = 0; k < 4; k++) v[k] = fg_tq_at(nzr[cc], k);
+            if (hov) {
+#pragma unroll
+                for (int k = 0; k < 2; k++)
+                    if (k < nh)
+                        v[k] = fg_med3((__mul24(hl[cc][k], k == 0 ? hwo0 : 17) + __mul24(v[k], k == 0 ? hwn0 : 27) + 16) >> 5,
+                                       gmin, gmax);
             }
-            const int32_t orig = org[r];
-            G<T> *d = out + (do_ + (uint32_t)dy * op);
+            if (c == 0 && rs < vrows) {  // the stripe's first rows: i == rs
+                int32_t o[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) o[k] = fg_tq_at(vor, k);
+                if (hov) {
+#pragma unroll
+                    for (int k = 0; k < 2; k++)
+                        if (k < nh)
+                            o[k] = fg_med3((__mul24(vl2[k], k == 0 ? hwo0 : 17) + __mul24(o[k], k == 0 ? hwn0 : 27) + 16) >> 5,
+                                           gmin, gmax);
+                }
+                const int32_t wo = psy ? 23 : (rs == 0 ? 27 : 17), wn = psy ? 22 : (rs == 0 ? 17 : 27);
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = fg_med3((__mul24(o[k], wo) + __mul24(v[k], wn) + 16) >> 5, gmin, gmax);
+            }
+            int32_t res[4];
             if (plane == 0) {
-                const int32_t n2 = fg_round2(fg_scale(lt, orig, bd) * nz, gs);
-                *d = (T)clip3(minv, maxl, orig + n2);
-                continue;
-            }
-            // chroma: scaled by the co-located pre-noise luma (film_grain.rs:757-808)
-            const int32_t avg = psx ? fg_round2(la[r] + lb[r], 1) : la[r];
-            int32_t merged;
-            if (from_luma) {
-                merged = avg;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int32_t orig = fg_at<T>(org[cc], k);
+                    const int32_t sc = fg_scale_at<T>(lt, orig, shift, rnds);
+                    res[k] = fg_med3(orig + ((fg_mul<T>(sc, v[k]) + rndg) >> gs), minv, maxl);
+                }
             } else {
-                const int32_t comb = avg * (lm - 128) + orig * (mm - 128);
-                merged = clip3(0, maxs, (comb >> 6) + coff);
+                // chroma: scaled by the co-located pre-noise luma
+                // (film_grain.rs:757-808): luma columns lx + 2k, lx + 2k + 1
+                // (lx + k without subsampling)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int32_t orig = fg_at<T>(org[cc], k);
+                    int32_t avg;
+                    if (psx) {
+                        const V &lv = k < 2 ? lar[cc] : lbr[cc];
+                        avg = (fg_at<T>(lv, 2 * (k & 1)) + fg_at<T>(lv, 2 * (k & 1) + 1) + 1) >> 1;
+                    } else {
+                        avg = fg_at<T>(lar[cc], k);
+                    }
+                    int32_t merged;
+                    if (from_luma) {
+                        merged = avg;
+                    } else {
+                        const int32_t comb = fg_mul<T>(avg, lm - 128) + fg_mul<T>(orig, mm - 128);
+                        merged = fg_med3((comb >> 6) + coff, 0, maxs);
+                    }
+                    const int32_t sc = fg_scale_at<T>(lt, merged, shift, rnds);
+                    res[k] = fg_med3(orig + ((fg_mul<T>(sc, v[k]) + rndg) >> gs), minv, maxc);
+                }
             }
-            const int32_t n2 = fg_round2(fg_scale(lt, merged, bd) * nz, gs);
-            *d = (T)clip3(minv, maxc, orig + n2);
+            fg_st4<T>(out + ((uint32_t)y * op + (uint32_t)x), vout, n, res);
         }
     }
 }
