@@ -44,18 +44,31 @@ struct CdefBlk {
     int16_t pri_y, sec_y, pri_uv, sec_uv;
 };
 
+// Samples per partial sum of the direction search: a diagonal of the 8x8
+// block holds min(k + 1, 15 - k) samples; the half-slope lines 2 * (the
+// number of rows they cross).
+constexpr int32_t kCdefCnt15[15] = {1, 2, 3, 4, 5, 6, 7, 8, 7, 6, 5, 4, 3, 2, 1};
+constexpr int32_t kCdefCnt11[11] = {2, 4, 6, 8, 8, 8, 8, 8, 6, 4, 2};
+
 // cdef_direction (cdef.rs:366-449), costs of two of the eight directions
 // for one 8x8 block read from the LDS tile: quarter 0 -> directions 0 and 2,
 // 1 -> 4 and 6, 2 -> 1 and 3, 3 -> 5 and 7 (stored at cost[d]).  Four threads
 // share a block so each keeps at most 26 partial sums in registers.  The
 // block rows and columns are clamped to the staged (buffer-bounded) extent
-// lh x lw, as the reference clamps its reads to the plane (cdef.rs:386-391).
-template <typename L, int Q>
+// lh x lw, as the reference clamps its reads to the plane (cdef.rs:386-391);
+// CLAMP = false for blocks inside it (constant LDS offsets).  The partial
+// sums add the shifted samples and take the -128 of every sample off once
+// per sum (each sum's sample count is fixed).
+template <typename L, int Q, bool CLAMP>
 __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
                                                  S3<int32_t> *cost) {
+    const S3<L> *base = t + (y8 + kCdefB) * ls + x8 + kCdefB;
     auto px = [&](int i, int j) -> int32_t {
-        const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
-        return ((int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift) - 128;
+        if (CLAMP) {
+            const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
+            return (int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift;
+        }
+        return (int32_t)base[i * ls + j] >> shift;
     };
     if constexpr (Q < 2) {
         // Q 0: d = 0 (i + j) and d = 2 (rows); Q 1: d = 4 (7 + i - j) and
@@ -63,17 +76,20 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
         // by Div_Table.
         int32_t pd[15], pa[8];
 #pragma unroll
-        for (int k = 0; k < 15; k++) pd[k] = 0;
+        for (int k = 0; k < 15; k++) pd[k] = -128 * kCdefCnt15[k];
 #pragma unroll
-        for (int k = 0; k < 8; k++) pa[k] = 0;
+        for (int k = 0; k < 8; k++) pa[k] = -128 * 8;
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 8; i++) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int32_t v = px(i, j);
                 pd[Q == 0 ? i + j : 7 + i - j] += v;
                 pa[Q == 0 ? i : j] += v;
             }
+            // one row's samples in flight at a time (registers)
+            __builtin_amdgcn_sched_barrier(0);
+        }
         int32_t cd = pd[7] * pd[7] * 105, ca = 0;
 #pragma unroll
         for (int k = 0; k < 7; k++) cd += (pd[k] * pd[k] + pd[14 - k] * pd[14 - k]) * c_div_table[k + 1];
@@ -87,15 +103,17 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
         // five weighted 105.
         int32_t pa[11], pb[11];
 #pragma unroll
-        for (int k = 0; k < 11; k++) pa[k] = pb[k] = 0;
+        for (int k = 0; k < 11; k++) pa[k] = pb[k] = -128 * kCdefCnt11[k];
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 8; i++) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int32_t v = px(i, j);
                 pa[Q == 2 ? i + j / 2 : 3 - i / 2 + j] += v;
                 pb[Q == 2 ? 3 + i - j / 2 : i / 2 + j] += v;
             }
+            __builtin_amdgcn_sched_barrier(0);
+        }
         auto half_cost = [&](const int32_t *p) {
             int32_t c = 0;
 #pragma unroll
@@ -113,11 +131,24 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
 template <typename L>
 __device__ __forceinline__ void cdef_dir_costs(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
                                                int quarter, S3<int32_t> *cost) {
+    const bool in = y8 + 8 <= lh && x8 + 8 <= lw;
     switch (quarter) {  // uniform per wave
-    case 0: cdef_dir_quarter<L, 0>(t, ls, lh, lw, y8, x8, shift, cost); break;
-    case 1: cdef_dir_quarter<L, 1>(t, ls, lh, lw, y8, x8, shift, cost); break;
-    case 2: cdef_dir_quarter<L, 2>(t, ls, lh, lw, y8, x8, shift, cost); break;
-    default: cdef_dir_quarter<L, 3>(t, ls, lh, lw, y8, x8, shift, cost); break;
+    case 0:
+        if (in) cdef_dir_quarter<L, 0, false>(t, ls, lh, lw, y8, x8, shift, cost);
+        else cdef_dir_quarter<L, 0, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        break;
+    case 1:
+        if (in) cdef_dir_quarter<L, 1, false>(t, ls, lh, lw, y8, x8, shift, cost);
+        else cdef_dir_quarter<L, 1, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        break;
+    case 2:
+        if (in) cdef_dir_quarter<L, 2, false>(t, ls, lh, lw, y8, x8, shift, cost);
+        else cdef_dir_quarter<L, 2, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        break;
+    default:
+        if (in) cdef_dir_quarter<L, 3, false>(t, ls, lh, lw, y8, x8, shift, cost);
+        else cdef_dir_quarter<L, 3, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        break;
     }
 }
 
@@ -263,13 +294,14 @@ typedef unsigned short cdef_u16x2 __attribute__((ext_vector_type(2)));
 // and - of each CdefTaps::off) from a pair's own dword in the first copy
 // (into the first copy for even element offsets, into the second for odd
 // ones), the strengths and damping shifts (capped at 15) and the primary
-// weights.
+// weights; bit 7 of pt0 says the block is filtered.  32 bytes, so a block's
+// entry is found with a shift.
 struct CdefTapsPk {
     int16_t boff[12];
     uint16_t pri, sec;
     uint8_t pri_adj, sec_adj, pt0, pt1;
-    uint8_t active, pad[3];
 };
+static_assert(sizeof(CdefTapsPk) == 32, "packed tap entries are 32 bytes");
 
 __device__ __forceinline__ CdefTapsPk cdef_taps_pk(const CdefTaps &t) {
     CdefTapsPk k{};
@@ -284,9 +316,8 @@ __device__ __forceinline__ CdefTapsPk cdef_taps_pk(const CdefTaps &t) {
     k.sec = (uint16_t)t.sec;
     k.pri_adj = (uint8_t)min((int32_t)t.pri_adj, 15);
     k.sec_adj = (uint8_t)min((int32_t)t.sec_adj, 15);
-    k.pt0 = t.pt0;
+    k.pt0 = (uint8_t)(t.pt0 | (t.active ? 0x80u : 0u));
     k.pt1 = t.pt1;
-    k.active = t.active;
     return k;
 }
 
@@ -320,7 +351,7 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
         const uint16_t *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
         const s16x2 x = pair_at(c, 0);
         s16x2 res = x;
-        if (b.active) {
+        if (b.pt0 & 0x80u) {
             const u16x2 thr_p = __builtin_bit_cast(u16x2, splat(b.pri));
             const u16x2 thr_s = __builtin_bit_cast(u16x2, splat(b.sec));
             const u16x2 adj_p = __builtin_bit_cast(u16x2, splat(b.pri_adj));
@@ -350,7 +381,7 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
                     mn = __builtin_elementwise_min(mn, v);
                 }
             }
-            const s16x2 w0 = __builtin_bit_cast(s16x2, splat(b.pt0));
+            const s16x2 w0 = __builtin_bit_cast(s16x2, splat(b.pt0 & 0x7fu));
             const s16x2 w1 = __builtin_bit_cast(s16x2, splat(b.pt1));
             const s16x2 sum = acc[0] * w0 + acc[1] * w1 + (acc[2] + acc[2]) + sec1;
             // (8 + sum - (sum < 0)) >> 4, arithmetic shifts on each half
@@ -381,21 +412,100 @@ __device__ __forceinline__ int div_magic(int i, uint32_t m) {
     return (int)(((uint32_t)i * m) >> 20);
 }
 
+// Stage a tile with its kCdefB border: t[yy * ls + xx] = src at (oy + yy -
+// 2, ox + xx - 2), both coordinates clamped to the plane (out-of-plane
+// entries are never used as taps; the clamp keeps every load in bounds).
+// DUAL: the u16 tile is written a second time kCdefDual elements on.  Where
+// the pitch allows, four plane columns (starting at a multiple of four) come
+// from one vector load and land with aligned dword stores.
 template <typename L, typename T, bool DUAL = false>
 __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int32_t oy, int32_t pw, int32_t ph,
                           const G<T> *__restrict__ src, uint32_t spitch) {
     const int ls = tw + 2 * kCdefB, lh = th + 2 * kCdefB;
-    const uint32_t m = ((1u << 20) + (uint32_t)ls - 1) / (uint32_t)ls;
-    const int n = ls * lh;
+    const bool vec = (spitch & 3u) == 0 && ((uintptr_t)src & (4 * sizeof(T) - 1)) == 0 && (ox & 3) == 0;
+    if (!vec) {
+        const uint32_t m = ((1u << 20) + (uint32_t)ls - 1) / (uint32_t)ls;
+        const int n = ls * lh;
 #pragma unroll 4
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int yy = div_magic(i, m), xx = i - yy * ls;
-        const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
-        // Out-of-plane entries are never used as taps; clamp keeps the load in bounds.
-        const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
-        const L v = (L)src[(size_t)yc * spitch + xc];
-        t[i] = v;
-        if (DUAL) t[kCdefDual + i] = v;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const int yy = div_magic(i, m), xx = i - yy * ls;
+            const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
+            const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
+            const L v = (L)src[(size_t)yc * spitch + xc];
+            t[i] = v;
+            if (DUAL) t[kCdefDual + i] = v;
+        }
+        return;
+    }
+    // group g covers plane columns ox - 4 + 4g .. +3, tile columns 4g - 2 .. 4g + 1
+    const int ng = (ls + 5) >> 2;
+    const uint32_t m = ((1u << 20) + (uint32_t)ng - 1) / (uint32_t)ng;
+    const int n = ng * lh;
+    constexpr int kR = 2;  // groups per thread and round (more raise the kernel's register use)
+    for (int i0 = 0; i0 < n; i0 += kR * (int)blockDim.x) {
+        int32_t v[kR][4];
+        int yr[kR], gr[kR];
+#pragma unroll
+        for (int r = 0; r < kR; r++) {
+            const int i = i0 + r * (int)blockDim.x + (int)threadIdx.x;
+            yr[r] = -1;
+            if (i < n) {
+                const int yy = div_magic(i, m), g = i - yy * ng;
+                yr[r] = yy;
+                gr[r] = g;
+                const G<T> *row = src + (size_t)clip3(0, ph - 1, oy + yy - kCdefB) * spitch;
+                const int32_t gx = ox - 4 + 4 * g;
+                if (gx >= 0 && gx + 3 < pw) {
+                    if constexpr (sizeof(T) == 1) {
+                        const uint32_t w = *(const G<uint32_t> *)(row + gx);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) v[r][k] = (int32_t)((w >> (8 * k)) & 255u);
+                    } else if constexpr (sizeof(T) == 2) {
+                        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+                        const u32x2 w = *(const G<u32x2> *)(row + gx);
+                        v[r][0] = (int32_t)(w.x & 0xffffu);
+                        v[r][1] = (int32_t)(w.x >> 16);
+                        v[r][2] = (int32_t)(w.y & 0xffffu);
+                        v[r][3] = (int32_t)(w.y >> 16);
+                    } else {
+                        typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+                        const i32x4 w = *(const G<i32x4> *)(row + gx);
+                        v[r][0] = w.x;
+                        v[r][1] = w.y;
+                        v[r][2] = w.z;
+                        v[r][3] = w.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) v[r][k] = (int32_t)row[clip3(0, pw - 1, gx + k)];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kR; r++) {
+            if (yr[r] < 0) continue;
+            const int x0 = 4 * gr[r] - kCdefB, e = yr[r] * ls + x0;
+            if constexpr (sizeof(L) == 2) {
+                if (x0 >= 0 && x0 + 3 < ls && !(e & 1)) {
+                    uint32_t *d = (uint32_t *)(t + e);
+                    d[0] = (uint32_t)v[r][0] | (uint32_t)v[r][1] << 16;
+                    d[1] = (uint32_t)v[r][2] | (uint32_t)v[r][3] << 16;
+                    if (DUAL) {  // kCdefDual is odd: e + kCdefDual + 1 is even
+                        L *q = t + kCdefDual + e;
+                        q[0] = (L)v[r][0];
+                        *(uint32_t *)(q + 1) = (uint32_t)v[r][1] | (uint32_t)v[r][2] << 16;
+                        q[3] = (L)v[r][3];
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (x0 + k >= 0 && x0 + k < ls) {
+                    t[e + k] = (L)v[r][k];
+                    if (DUAL) t[kCdefDual + e + k] = (L)v[r][k];
+                }
+        }
     }
 }
 
