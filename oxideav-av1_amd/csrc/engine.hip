@@ -87,7 +87,7 @@ __device__ __forceinline__ void cdef_item(const FrameJob &J, uint32_t local, typ
 // four rows around each stripe boundary k >= 1: rows k*(64>>sy) - (8>>sy) - 2
 // .. +1 (the rows loop restoration reads from it).
 template <typename T>
-__device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local) {
+__device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local, const int16_t *ftab) {
     const uint32_t p6 = plane_of(J.item_base[kStageSr], 6, local);
     const uint32_t chunk = local - J.item_base[kStageSr][p6];
     const uint32_t p = p6 % 3;
@@ -108,7 +108,8 @@ __device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local) {
     sr_rows<T>(wave_ptr(in), wave_u32(ip), (int32_t)wave_u32(J.pad_cols[p]), wave_ptr(out), wave_u32(op),
                (int32_t)wave_u32(J.out_cols[p]), (int32_t)wave_u32((uint32_t)y0), (int32_t)wave_u32((uint32_t)nrows),
                (int32_t)wave_u32(J.out_rows[p]), (int32_t)wave_u32((uint32_t)J.sr_step[p]),
-               (int32_t)wave_u32((uint32_t)J.sr_init[p]), (int32_t)wave_u32((uint32_t)J.sr_maxx[p]), (int32_t)wave_u32(J.bd));
+               (int32_t)wave_u32((uint32_t)J.sr_init[p]), (int32_t)wave_u32((uint32_t)J.sr_maxx[p]), (int32_t)wave_u32(J.bd),
+               ftab);
 }
 
 template <typename T>
@@ -263,14 +264,18 @@ __device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *
 template <int KIND>
 __device__ __forceinline__ void sr_body(const FrameJob *jobs, const uint32_t *prefix, uint32_t njobs,
                                                  uint32_t total, uint32_t *counter, uint32_t claim) {
+    // the upscale filter table, staged once per workgroup
+    __shared__ __align__(16) int16_t ftab[64 * 8];
+    for (int i = threadIdx.x; i < 64 * 8; i += blockDim.x) ftab[i] = c_upscale_filter[i >> 3][i & 7];
+    __syncthreads();
     OXAV1_ITEM_LOOP({
         if constexpr (KIND == 1) {
-            sr_item<int32_t>(J, ref.local);
+            sr_item<int32_t>(J, ref.local, ftab);
         } else {
             if (J.kind == kU8)
-                sr_item<uint8_t>(J, ref.local);
+                sr_item<uint8_t>(J, ref.local, ftab);
             else
-                sr_item<uint16_t>(J, ref.local);
+                sr_item<uint16_t>(J, ref.local, ftab);
         }
     })
 }
