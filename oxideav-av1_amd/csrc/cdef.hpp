@@ -27,14 +27,8 @@ __constant__ int8_t c_cdef_dir[8][2][2] = {
     {{1, 1}, {2, 2}},   {{1, 0}, {2, 1}},  {{1, 0}, {2, 0}}, {{1, 0}, {2, -1}}};
 __constant__ uint8_t c_cdef_uv_dir[2][2][8] = {{{0, 1, 2, 3, 4, 5, 6, 7}, {1, 2, 2, 2, 3, 4, 6, 0}},
                                                {{7, 0, 2, 4, 5, 6, 6, 6}, {0, 1, 2, 3, 4, 5, 6, 7}}};
-__constant__ int32_t c_div_table[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+constexpr uint32_t kCdefDiv[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};  // Div_Table
 
-// Median of three signed integers in one VALU instruction (v_med3_i32).
-__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
-    int32_t r;
-    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
 
 // Per 8x8 block decisions shared by the unit's threads.
 struct CdefBlk {
@@ -59,10 +53,22 @@ constexpr int32_t kCdefCnt11[11] = {2, 4, 6, 8, 8, 8, 8, 8, 6, 4, 2};
 // CLAMP = false for blocks inside it (constant LDS offsets).  The partial
 // sums add the shifted samples and take the -128 of every sample off once
 // per sum (each sum's sample count is fixed).
+// x * K; K24: 0 <= x < 2^24, on the 24-bit multiplier.
+template <bool K24, uint32_t K>
+__device__ __forceinline__ int32_t cdef_kmul(int32_t x) {
+    return K24 ? (int32_t)mul_u24k<K>((uint32_t)x) : x * (int32_t)K;
+}
+
 template <typename L, int Q, bool CLAMP>
 __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
                                                  S3<int32_t> *cost) {
-    const S3<L> *base = t + (y8 + kCdefB) * ls + x8 + kCdefB;
+    const S3<L> *base = t + mul_u24(y8 + kCdefB, ls) + x8 + kCdefB;
+    // With 16-bit tiles the partial sums hold at most eight samples of
+    // 8 bits less 128 (|sum| <= 1024), so squares (<= 2^20), pairs of them
+    // times Div_Table (< 2^31) and the eight-square sums times 105 (< 2^31)
+    // run on the 24-bit multiplier; int32 tiles keep 32-bit products.
+    constexpr bool k24 = sizeof(L) <= 2;
+    auto sq = [](int32_t p, int32_t acc) -> int32_t { return k24 ? mad_i24(p, p, acc) : p * p + acc; };
     auto px = [&](int i, int j) -> int32_t {
         if (CLAMP) {
             const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
@@ -90,13 +96,19 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
             // one row's samples in flight at a time (registers)
             __builtin_amdgcn_sched_barrier(0);
         }
-        int32_t cd = pd[7] * pd[7] * 105, ca = 0;
+        // Div_Table[k + 1] weighs lines k and 14 - k
+        int32_t cd = cdef_kmul<k24, 105>(sq(pd[7], 0)), ca = 0;
+        cd += cdef_kmul<k24, kCdefDiv[1]>(sq(pd[0], sq(pd[14], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[2]>(sq(pd[1], sq(pd[13], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[3]>(sq(pd[2], sq(pd[12], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[4]>(sq(pd[3], sq(pd[11], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[5]>(sq(pd[4], sq(pd[10], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[6]>(sq(pd[5], sq(pd[9], 0)));
+        cd += cdef_kmul<k24, kCdefDiv[7]>(sq(pd[6], sq(pd[8], 0)));
 #pragma unroll
-        for (int k = 0; k < 7; k++) cd += (pd[k] * pd[k] + pd[14 - k] * pd[14 - k]) * c_div_table[k + 1];
-#pragma unroll
-        for (int k = 0; k < 8; k++) ca += pa[k] * pa[k];
+        for (int k = 0; k < 8; k++) ca = sq(pa[k], ca);
         cost[Q == 0 ? 0 : 4] = cd;
-        cost[Q == 0 ? 2 : 6] = ca * 105;
+        cost[Q == 0 ? 2 : 6] = cdef_kmul<k24, 105>(ca);
     } else {
         // Q 2: d = 1 (i + j/2) and d = 3 (3 + i - j/2); Q 3: d = 5
         // (3 - i/2 + j) and d = 7 (i/2 + j).  11 partial sums, the middle
@@ -117,10 +129,11 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
         auto half_cost = [&](const int32_t *p) {
             int32_t c = 0;
 #pragma unroll
-            for (int k = 0; k < 5; k++) c += p[3 + k] * p[3 + k];
-            c *= 105;
-#pragma unroll
-            for (int k = 0; k < 3; k++) c += (p[k] * p[k] + p[10 - k] * p[10 - k]) * c_div_table[2 * k + 2];
+            for (int k = 0; k < 5; k++) c = sq(p[3 + k], c);
+            c = cdef_kmul<k24, 105>(c);
+            c += cdef_kmul<k24, kCdefDiv[2]>(sq(p[0], sq(p[10], 0)));
+            c += cdef_kmul<k24, kCdefDiv[4]>(sq(p[1], sq(p[9], 0)));
+            c += cdef_kmul<k24, kCdefDiv[6]>(sq(p[2], sq(p[8], 0)));
             return c;
         };
         cost[Q == 2 ? 1 : 5] = half_cost(pa);
@@ -238,7 +251,7 @@ __device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int
         const int32_t py = oy + yy;
         if (py >= ph) break;
         const CdefTaps b = taps[(yy >> bh_log2) * 8 + (xx >> bw_log2)];
-        const L *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
+        const L *c = t + mul_u24(yy + kCdefB, ls) + (xx + kCdefB);
         const int32_t x = (int32_t)c[0];
         int32_t out = x;
         if (b.active) {
@@ -264,7 +277,9 @@ __device__ __forceinline__ void cdef_filter_tile(const L *t, int ls, int tw, int
                     else
                         ad = (uint32_t)abs(d);
                     const int32_t tt = (int32_t)__builtin_elementwise_sub_sat(thr, ad >> adj);
-                    sum += w * med3_i32(d, -tt, tt);  // d clamped to [-tt, tt], tt >= 0
+                    // d clamped to [-tt, tt], tt >= 0; |d| < 2^16 for 16-bit tiles
+                    const int32_t dc = med3_i32(d, -tt, tt);
+                    sum = sizeof(L) <= 2 ? mad_i24(w, dc, sum) : sum + w * dc;
                     mx = max(mx, v);
                     mn = min(mn, v);
                 }
@@ -345,10 +360,10 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
     auto pair_at = [](const uint16_t *q, int32_t byte_off) -> s16x2 {
         return __builtin_bit_cast(s16x2, *(const uint32_t *)((const char *)q + byte_off));
     };
-    auto splat = [](int32_t v) -> uint32_t { return (uint32_t)v * 0x10001u; };
+    auto splat = [](int32_t v) -> uint32_t { return (uint32_t)v | (uint32_t)v << 16; };  // 0 <= v < 2^16
     for (int yy = y_first; yy < th; yy += ystep) {
         const CdefTapsPk b = taps[(yy >> bh_log2) * 8 + (xx >> bw_log2)];
-        const uint16_t *c = t + (yy + kCdefB) * ls + (xx + kCdefB);
+        const uint16_t *c = t + mul_u24(yy + kCdefB, ls) + (xx + kCdefB);
         const s16x2 x = pair_at(c, 0);
         s16x2 res = x;
         if (b.pt0 & 0x80u) {
@@ -409,7 +424,7 @@ __device__ __forceinline__ void cdef_filter_plane(const L *t, int ls, int tw, in
 
 // Exact i / d for 0 <= i < 2^20 / d with m = ceil(2^20 / d), d <= 128.
 __device__ __forceinline__ int div_magic(int i, uint32_t m) {
-    return (int)(((uint32_t)i * m) >> 20);
+    return (int)(mul_u24((uint32_t)i, m) >> 20);
 }
 
 // Stage a tile with its kCdefB border: t[yy * ls + xx] = src at (oy + yy -
@@ -428,7 +443,7 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
         const int n = ls * lh;
 #pragma unroll 4
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int yy = div_magic(i, m), xx = i - yy * ls;
+            const int yy = div_magic(i, m), xx = i - (int)mul_u24(yy, ls);
             const int32_t y = oy + yy - kCdefB, x = ox + xx - kCdefB;
             const int32_t yc = clip3(0, ph - 1, y), xc = clip3(0, pw - 1, x);
             const L v = (L)src[(size_t)yc * spitch + xc];
@@ -450,7 +465,7 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
             const int i = i0 + r * (int)blockDim.x + (int)threadIdx.x;
             yr[r] = -1;
             if (i < n) {
-                const int yy = div_magic(i, m), g = i - yy * ng;
+                const int yy = div_magic(i, m), g = i - (int)mul_u24(yy, ng);
                 yr[r] = yy;
                 gr[r] = g;
                 const G<T> *row = src + (size_t)clip3(0, ph - 1, oy + yy - kCdefB) * spitch;
@@ -484,7 +499,7 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
 #pragma unroll
         for (int r = 0; r < kR; r++) {
             if (yr[r] < 0) continue;
-            const int x0 = 4 * gr[r] - kCdefB, e = yr[r] * ls + x0;
+            const int x0 = 4 * gr[r] - kCdefB, e = (int)mul_u24(yr[r], ls) + x0;
             if constexpr (sizeof(L) == 2) {
                 if (x0 >= 0 && x0 + 3 < ls && !(e & 1)) {
                     uint32_t *d = (uint32_t *)(t + e);

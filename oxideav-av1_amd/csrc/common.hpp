@@ -116,6 +116,43 @@ enum Stage : int { kStageLf = 0, kStageCdef = 1, kStageSr = 2, kStageLr = 3, kSt
 __device__ __forceinline__ int32_t clip3(int32_t a, int32_t b, int32_t x) {
     return x < a ? a : (x > b ? b : x);
 }
+// Median of three signed integers in one VALU instruction (v_med3_i32); with
+// lo <= hi, med3_i32(v, lo, hi) is v clamped to [lo, hi].
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Products of operands that fit 24 bits (two's complement for the i24 forms,
+// unsigned for the u24 forms) as single full-rate instructions: the low 32
+// bits of x * y (+ z).  The compiler picks these by itself only where it can
+// derive both operand ranges; for sums of many samples, products of products
+// and table values it falls back to the quarter-rate v_mul_lo_u32.
+__device__ __forceinline__ int32_t mul_i24(int32_t x, int32_t y) {
+    int32_t r;
+    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+}
+__device__ __forceinline__ int32_t mad_i24(int32_t x, int32_t y, int32_t z) {
+    int32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(z));
+    return r;
+}
+__device__ __forceinline__ uint32_t mul_u24(uint32_t x, uint32_t y) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+}
+// x * K for a constant K < 2^24, K as the instruction's literal operand.
+template <uint32_t K>
+__device__ __forceinline__ uint32_t mul_u24k(uint32_t x) {
+    static_assert(K < (1u << 24), "24-bit constant");
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "n"(K), "v"(x));
+    return r;
+}
+
 __device__ __forceinline__ int32_t round2(int32_t x, uint32_t n) {
     return n == 0 ? x : (x + (1 << (n - 1))) >> n;
 }

@@ -438,7 +438,9 @@ static int ensure_device_init() {
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_max_tx_rect), kMaxTxSizeRect, sizeof(kMaxTxSizeRect)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_upscale_filter), kUpscaleFilter, sizeof(kUpscaleFilter)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_params), kSgrParams, sizeof(kSgrParams)));
-    uint16_t a2[256] = {0};
+    uint16_t a2[256];
+    a2[0] = 1;
+    a2[255] = 256;
     for (int z = 1; z < 255; z++) a2[z] = (uint16_t)(((z << 8) + z / 2) / (z + 1));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_a2), a2, sizeof(a2)));
     LrSgrConsts sc[16][2];

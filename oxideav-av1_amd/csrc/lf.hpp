@@ -206,17 +206,24 @@ template <typename L, bool EDGE>
 struct LfRegion {
     L *s;
     int32_t ry0, rx0, rows, cols;
+    // (y, x) inside the region: 0 <= y - ry0 < kLfRegion (said to the
+    // compiler, which then forms the row offset on the 24-bit multiplier)
+    __device__ __forceinline__ int at(int32_t y, int32_t x) const {
+        const int32_t dy = y - ry0;
+        __builtin_assume((uint32_t)dy < (uint32_t)kLfRegion);
+        return dy * kLfStride + (x - rx0);
+    }
     __device__ __forceinline__ int idx(int32_t y, int32_t x) const {
         if (EDGE) {
             y = clip3(0, rows - 1, y);
             x = clip3(0, cols - 1, x);
         }
-        return (y - ry0) * kLfStride + (x - rx0);
+        return at(y, x);
     }
     __device__ __forceinline__ int32_t get(int32_t y, int32_t x) const { return (int32_t)s[idx(y, x)]; }
     __device__ __forceinline__ void set(int32_t y, int32_t x, int32_t v) const {
         if (EDGE && (y < 0 || x < 0 || y >= rows || x >= cols)) return;
-        s[(y - ry0) * kLfStride + (x - rx0)] = (L)v;
+        s[at(y, x)] = (L)v;
     }
 };
 

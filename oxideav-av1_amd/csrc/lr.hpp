@@ -48,12 +48,10 @@ __host__ __device__ inline uint64_t lr_magic(uint32_t d) { return (((uint64_t)1 
 __device__ __forceinline__ uint32_t lr_mdiv(uint32_t n, uint64_t m) { return (uint32_t)(((uint64_t)n * m) >> 40); }
 __constant__ uint64_t c_lr_recip[130];      // lr_magic(k), k = 1..129
 
-// x * y for |x|, |y| < 2^23: the low 32 bits of the product (the same bits
-// a wrapping 32-bit multiply gives) on the full-rate 24-bit multiplier.
-__device__ __forceinline__ int32_t lr_mul24(int32_t x, int32_t y) { return __mul24(x, y); }
 
 __constant__ int32_t c_sgr_params[16][4];   // filled from kSgrParams at init
-__constant__ uint16_t c_sgr_a2[256];        // a2 for 0 < z < 255 (loop_restoration.rs:648-654)
+__constant__ uint16_t c_sgr_a2[256];        // a2 of z = 0 .. 255 (loop_restoration.rs:648-654)
+
 
 // A (<= 256, 9 bits) and B (< 2^20) of one self-guided position in one word.
 __device__ __forceinline__ uint32_t lr_pack_ab(uint32_t a, uint32_t b) { return (a << 20) | b; }
@@ -92,18 +90,23 @@ struct LrTileGeom {
 template <bool K24>
 __device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, int32_t n, uint32_t s, uint32_t one_over_n,
                                           uint32_t bd, const uint16_t *a2tab) {
-    auto mul = [](int32_t x, int32_t y) -> int32_t { return K24 ? lr_mul24(x, y) : x * y; };
+    auto mul = [](int32_t x, int32_t y) -> int32_t { return K24 ? mul_i24(x, y) : x * y; };
     const uint32_t sha = 2 * (bd - 8), shb = bd - 8;
     const uint32_t ar = sha ? ((a + (1u << (sha - 1))) >> sha) : a;
     const int32_t dr = shb ? ((b + (1 << (shb - 1))) >> shb) : b;
-    // ar < 2^21 (at most 25 squares of 12-bit samples, shifted by 8), n <= 25
-    // and |dr| <= 25 * 255 + 1 at every depth: 24-bit multiplies
+    // p = ar * n - dr^2 clipped at 0 is at most n^2 / 4 squared sample
+    // ranges (two-valued windows; 1.311e6 for r = 1, 10.23e6 for r = 2 at
+    // every depth, rounding included), and s <= 3236 for r = 1, s <= 140 for
+    // r = 2: p * s < 2^32 (4.25e9 at most), as the reference's saturating
+    // 64-bit product never saturates.  K24: ar < 2^21, dr <= 25 * 255 + 1
+    // and p < 2^24, so all three products run on the 24-bit multiplier.
     int32_t p = mul((int32_t)ar, n) - mul(dr, dr);
     p = p < 0 ? 0 : p;
-    // p * s < 2^32 at every depth (p < 41.2e6 with s <= 76 for r = 2, p < 5.3e6
-    // with s <= 15 for r = 1)
-    const uint32_t z = ((uint32_t)p * s + (1u << 19)) >> 20;
-    const uint32_t a2 = z >= 255 ? 256u : (z == 0 ? 1u : (uint32_t)a2tab[z]);
+    const uint32_t ps = K24 ? mul_u24((uint32_t)p, s) : (uint32_t)p * s;
+    const uint32_t z = (ps + (1u << 19)) >> 20;
+    // the table holds 1 at z = 0 and 256 at z = 255, so every z >= 255 reads
+    // its last entry
+    const uint32_t a2 = a2tab[z < 255u ? z : 255u];
     // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth;
     // (256 - a2) * oneOverN <= 255 * 455 and b < 2^17, so both factors run on
     // the 24-bit multiplier
@@ -382,10 +385,12 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                 const L *q = S.src + 1 + wr * kLrSw + lane;
                 // u16 samples: |taps| < 2^9 and sample sums < 2^17, 24-bit multiplies
                 constexpr bool k24h = std::is_same<L, uint16_t>::value;
-                auto hm = [](int32_t x, int32_t y) -> int32_t { return k24h ? lr_mul24(x, y) : x * y; };
-                const int32_t acc = hm(hf[0], (int32_t)q[0] + (int32_t)q[6]) + hm(hf[1], (int32_t)q[1] + (int32_t)q[5]) +
-                                    hm(hf[2], (int32_t)q[2] + (int32_t)q[4]) + hm(hf[3], (int32_t)q[3]);
-                S.inter[wr * kLrMaxW + lane] = clip3(-offset, limit - offset, (acc + (1 << (r0 - 1))) >> r0);
+                auto hm = [](int32_t x, int32_t y, int32_t z) -> int32_t { return k24h ? mad_i24(x, y, z) : x * y + z; };
+                const int32_t acc = hm(hf[0], (int32_t)q[0] + (int32_t)q[6],
+                                       hm(hf[1], (int32_t)q[1] + (int32_t)q[5],
+                                          hm(hf[2], (int32_t)q[2] + (int32_t)q[4],
+                                             hm(hf[3], (int32_t)q[3], 1 << (r0 - 1)))));
+                S.inter[wr * kLrMaxW + lane] = med3_i32(acc >> r0, -offset, limit - offset);
             }
         }
         __syncthreads();
@@ -402,9 +407,11 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
 #pragma unroll
             for (int i = 0; i < kLrBand; i++)
                 if (rb + i < b1) {
-                    const int32_t acc = lr_mul24(vf[0], h[i] + h[i + 6]) + lr_mul24(vf[1], h[i + 1] + h[i + 5]) +
-                                        lr_mul24(vf[2], h[i + 2] + h[i + 4]) + lr_mul24(vf[3], h[i + 3]);
-                    sink.put(g.ys + rb + i, g.xs + lane, clip3(0, maxv, (acc + (1 << (r1 - 1))) >> r1));
+                    const int32_t acc =
+                        mad_i24(vf[0], h[i] + h[i + 6],
+                                mad_i24(vf[1], h[i + 1] + h[i + 5],
+                                        mad_i24(vf[2], h[i + 2] + h[i + 4], mad_i24(vf[3], h[i + 3], 1 << (r1 - 1)))));
+                    sink.put(g.ys + rb + i, g.xs + lane, med3_i32(acc >> r1, 0, maxv));
                 }
         }
         return;
@@ -520,7 +527,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     // (A sums <= 8192, samples < 2^16, filter outputs < 2^22, weights from
     // int16 xqd), so it runs on the 24-bit multiplier.
     constexpr bool k24 = std::is_same<L, uint16_t>::value;
-    auto mul = [](int32_t x, int32_t y) -> int32_t { return k24 ? lr_mul24(x, y) : x * y; };
+    auto mad = [](int32_t x, int32_t y, int32_t z) -> int32_t { return k24 ? mad_i24(x, y, z) : x * y + z; };
     for (int kb = b0; kb < b1; kb += kLrBand) {
 #pragma unroll
         for (int i = 0; i < kLrBand; i++) {
@@ -545,7 +552,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                     pb_prev = pb_next;
                     sh = 5;
                 }
-                const int32_t v = mul(as, uval) + bs;
+                const int32_t v = mad(as, uval, bs);
                 const uint32_t ts = 8 + sh - 4;
                 f0 = (v + (1 << (ts - 1))) >> ts;
             }
@@ -554,16 +561,14 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                 row3(S.ab1, (k + 2) * kLrAw, sa2, ma2, sb2, mb2);
                 const int32_t as = 3 * (sa0 + sa2) + ma0 + ma2 + 4 * sa1;
                 const int32_t bs = 3 * (sb0 + sb2) + mb0 + mb2 + 4 * sb1;
-                const int32_t v = mul(as, uval) + bs;
+                const int32_t v = mad(as, uval, bs);
                 f1 = (v + (1 << 8)) >> 9;
                 sa0 = sa1, ma0 = ma1, sb0 = sb1, mb0 = mb1;
                 sa1 = sa2, ma1 = ma2, sb1 = sb2, mb1 = mb2;
             }
-            int32_t v = mul(w1, u4);
-            v += mul(w0, r0 != 0 ? f0 : u4);
-            v += mul(w2, r1 != 0 ? f1 : u4);
-            const int32_t sres = (v + (1 << 10)) >> 11;
-            sink.sgr(ys + k, g.xs + lane, clip3(0, maxv, sres), uval, f0, f1);
+            const int32_t v = mad(w2, r1 != 0 ? f1 : u4, mad(w0, r0 != 0 ? f0 : u4, mad(w1, u4, 1 << 10)));
+            const int32_t sres = v >> 11;
+            sink.sgr(ys + k, g.xs + lane, med3_i32(sres, 0, maxv), uval, f0, f1);
         }
     }
 }
