@@ -144,6 +144,19 @@ __device__ __forceinline__ uint32_t mul_u24(uint32_t x, uint32_t y) {
     asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
     return r;
 }
+// The same with the first factor k wave-uniform, read from a scalar
+// register (k must not vary across the wave: a varying value would be
+// replaced by its first active lane's).
+__device__ __forceinline__ int32_t mad_i24_s(int32_t k, int32_t x, int32_t z) {
+    int32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "s"(k), "v"(x), "v"(z));
+    return r;
+}
+__device__ __forceinline__ uint32_t mul_u24_s(uint32_t k, uint32_t x) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(k), "v"(x));
+    return r;
+}
 // x * K for a constant K < 2^24, K as the instruction's literal operand.
 template <uint32_t K>
 __device__ __forceinline__ uint32_t mul_u24k(uint32_t x) {

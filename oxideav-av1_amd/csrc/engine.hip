@@ -443,11 +443,11 @@ static int ensure_device_init() {
     a2[255] = 256;
     for (int z = 1; z < 255; z++) a2[z] = (uint16_t)(((z << 8) + z / 2) / (z + 1));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_a2), a2, sizeof(a2)));
-    LrSgrConsts sc[16][2];
+    uint32_t sgr_s[16][2];
     for (int set = 0; set < 16; set++)
         for (int pass = 0; pass < 2; pass++)
-            sc[set][pass] = lr_sgr_consts_of(kSgrParams[set * 4 + 2 * pass], kSgrParams[set * 4 + 2 * pass + 1]);
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_consts), sc, sizeof(sc)));
+            sgr_s[set][pass] = lr_sgr_s_of(kSgrParams[set * 4 + 2 * pass], kSgrParams[set * 4 + 2 * pass + 1]);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sgr_s), sgr_s, sizeof(sgr_s)));
     uint64_t recip[130];
     for (uint32_t k = 0; k < 130; k++) recip[k] = lr_magic(k);
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_lr_recip), recip, sizeof(recip)));

@@ -39,6 +39,7 @@ constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // pass-0 A/B rows: the odd rows onl
 // windows need no rotating copies.
 constexpr int kLrBand = (kLrMaxH + 3) / 4;
 constexpr int kLrAbBand = (kLrMaxH + 2 + 2) / 3;
+static_assert(kLrAbBand % 2 == 0, "A/B row steps keep the row parity");
 
 // n / d for n < 2^24 and 1 <= d < 2^16 as one multiply and shift: with
 // m = floor(2^40 / d) + 1 = 2^40 / d + e, 0 < e <= 1, n * m / 2^40 is
@@ -87,52 +88,51 @@ struct LrTileGeom {
 // K24: samples are u8/u16 (the engine's frames), so the bounds below hold
 // and the products run on the 24-bit multiplier; the drop-in path's int32
 // samples keep 32-bit multiplies.
-template <bool K24>
-__device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, int32_t n, uint32_t s, uint32_t one_over_n,
-                                          uint32_t bd, const uint16_t *a2tab) {
-    auto mul = [](int32_t x, int32_t y) -> int32_t { return K24 ? mul_i24(x, y) : x * y; };
+template <bool K24, int32_t N>
+__device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, uint32_t s, uint32_t bd, const uint16_t *a2tab) {
+    constexpr uint32_t kOon = ((1u << 12) + N / 2) / N;  // oneOverN
     const uint32_t sha = 2 * (bd - 8), shb = bd - 8;
-    const uint32_t ar = sha ? ((a + (1u << (sha - 1))) >> sha) : a;
-    const int32_t dr = shb ? ((b + (1 << (shb - 1))) >> shb) : b;
+    // Round2 by sha / shb (a shift by 0 adds 0)
+    const uint32_t ar = (a + ((1u << sha) >> 1)) >> sha;
+    const int32_t dr = (b + ((1 << shb) >> 1)) >> shb;
     // p = ar * n - dr^2 clipped at 0 is at most n^2 / 4 squared sample
     // ranges (two-valued windows; 1.311e6 for r = 1, 10.23e6 for r = 2 at
     // every depth, rounding included), and s <= 3236 for r = 1, s <= 140 for
     // r = 2: p * s < 2^32 (4.25e9 at most), as the reference's saturating
-    // 64-bit product never saturates.  K24: ar < 2^21, dr <= 25 * 255 + 1
-    // and p < 2^24, so all three products run on the 24-bit multiplier.
-    int32_t p = mul((int32_t)ar, n) - mul(dr, dr);
-    p = p < 0 ? 0 : p;
-    const uint32_t ps = K24 ? mul_u24((uint32_t)p, s) : (uint32_t)p * s;
-    const uint32_t z = (ps + (1u << 19)) >> 20;
-    // the table holds 1 at z = 0 and 256 at z = 255, so every z >= 255 reads
-    // its last entry
-    const uint32_t a2 = a2tab[z < 255u ? z : 255u];
-    // b2 = (256 - a2) * b * oneOverN is non-negative and < 2^32 at every depth;
-    // (256 - a2) * oneOverN <= 255 * 455 and b < 2^17, so both factors run on
-    // the 24-bit multiplier
-    const uint32_t b2 = (uint32_t)mul(mul((int32_t)(256u - a2), (int32_t)one_over_n), b);
+    // 64-bit product never saturates.  (256 - a2) * oneOverN <= 255 * 455
+    // and b < 2^17, so b2 < 2^32.  K24: ar < 2^21, 0 <= dr <= 25 * 255 + 1
+    // and p < 2^24, so every product runs on the 24-bit multiplier.
+    uint32_t a2, b2;
+    if constexpr (K24) {
+        int32_t p = (int32_t)mul_u24k<N>(ar) - (int32_t)mul_u24((uint32_t)dr, (uint32_t)dr);
+        p = p < 0 ? 0 : p;
+        const uint32_t z = (mul_u24_s(s, (uint32_t)p) + (1u << 19)) >> 20;
+        // the table holds 1 at z = 0 and 256 at z = 255, so every z >= 255
+        // reads its last entry
+        a2 = a2tab[z < 255u ? z : 255u];
+        b2 = mul_u24(mul_u24k<kOon>(256u - a2), (uint32_t)b);
+    } else {
+        int32_t p = (int32_t)ar * N - dr * dr;
+        p = p < 0 ? 0 : p;
+        const uint32_t z = ((uint32_t)p * s + (1u << 19)) >> 20;
+        a2 = a2tab[z < 255u ? z : 255u];
+        b2 = (256u - a2) * kOon * (uint32_t)b;
+    }
     return lr_pack_ab(a2, (b2 + 2048u) >> 12);
 }
 
-// n, s = Round2-style (2^20 + n^2 e / 2) / (n^2 e) and oneOverN =
-// (2^12 + n / 2) / n of one self-guided pass (loop_restoration.rs:620-640),
-// per (set, pass): they depend on Sgr_Params only, so the host computes
-// them once at device set-up instead of every thread per work item (a 64-bit
-// division each).
-struct LrSgrConsts {
-    int32_t n;
-    uint32_t s, oon;
-};
-__constant__ LrSgrConsts c_sgr_consts[16][2];
+// s = (2^20 + n^2 e / 2) / (n^2 e) of one self-guided pass
+// (loop_restoration.rs:612-616) per (set, pass): it depends on Sgr_Params
+// only, so the host computes it once at device set-up instead of every
+// thread per work item (a 64-bit division each); n and oneOverN follow from
+// the radius, which the kernels know at compile time.
+__constant__ uint32_t c_sgr_s[16][2];
 
-__host__ __device__ inline LrSgrConsts lr_sgr_consts_of(int32_t r, int32_t e) {
-    LrSgrConsts c{};
-    if (r == 0) return c;
-    c.n = (2 * r + 1) * (2 * r + 1);
-    const int64_t n2e = (int64_t)c.n * c.n * e;
-    c.s = (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
-    c.oon = (uint32_t)(((1 << 12) + c.n / 2) / c.n);
-    return c;
+__host__ __device__ inline uint32_t lr_sgr_s_of(int32_t r, int32_t e) {
+    if (r == 0) return 0;
+    const int64_t n = (2 * r + 1) * (2 * r + 1);
+    const int64_t n2e = n * n * e;
+    return (uint32_t)((((int64_t)1 << 20) + n2e / 2) / (n2e > 1 ? n2e : 1));
 }
 
 // unit(): the tile's restoration unit (called once the unit is resolved);
@@ -261,17 +261,27 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
         if (!J.lr_copy_none) return;
         if constexpr (Sink::kCopy4) {
           if (sink.copy4_ok() && (dpitch & 3u) == 0 && ((uintptr_t)cdf & (4 * sizeof(T) - 1)) == 0) {
-            // four samples per lane (the tile starts at a multiple of 32)
-            const int gw = (W + 3) >> 2;
-            const uint64_t mg = c_lr_recip[gw];
-            for (int i = tid; i < H * gw; i += nt) {
-                const int r = (int)lr_mdiv((uint32_t)i, mg), gi = i - r * gw;
-                const int32_t y = g.ys + r, x = g.xs + 4 * gi;
-                const G<T> *q = cdf + (size_t)y * dpitch + x;
-                if (x + 3 <= g.xe) {
-                    sink.copy4(y, x, *(const G<V> *)q);
-                } else {
-                    for (int k = 0; k < 4 && x + k <= g.xe; k++) sink.put(y, x + k, (int32_t)q[k]);
+            // four samples per lane (the tile starts at a multiple of 32):
+            // thread t copies group t % 16 of rows t / 16 and t / 16 + 16 (a
+            // tile is at most 16 groups wide and 32 rows high), both loads
+            // issued before either store
+            static_assert(kLrMaxW == 64 && kLrMaxH == 32 && kLrWaves == 4, "copy mapping");
+            const int gi = tid & 15, r = tid >> 4;
+            const int32_t x = g.xs + 4 * gi;
+            if (x > g.xe) return;
+            const G<T> *q0 = cdf + (size_t)(g.ys + r) * dpitch + x;
+            const G<T> *q1 = q0 + (size_t)16 * dpitch;
+            const bool has0 = r < H, has1 = r + 16 < H;
+            if (x + 3 <= g.xe) {
+                V v0{}, v1{};
+                if (has0) v0 = *(const G<V> *)q0;
+                if (has1) v1 = *(const G<V> *)q1;
+                if (has0) sink.copy4(g.ys + r, x, v0);
+                if (has1) sink.copy4(g.ys + r + 16, x, v1);
+            } else {
+                for (int k = 0; k < 4 && x + k <= g.xe; k++) {
+                    if (has0) sink.put(g.ys + r, x + k, (int32_t)q0[k]);
+                    if (has1) sink.put(g.ys + r + 16, x + k, (int32_t)q1[k]);
                 }
             }
             return;
@@ -364,7 +374,8 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     const int32_t maxv = (1 << bd) - 1;
     if (type == OXAV1_RESTORE_WIENER) {
         // wiener_filter (loop_restoration.rs:742-821); both 7-tap filters are
-        // symmetric, so taps t and 6-t share one multiply
+        // symmetric, so taps t and 6-t share one multiply; the taps are the
+        // unit's (wave-uniform), multiplied from scalar registers
         int32_t vf[4], hf[4];
         vf[3] = hf[3] = 128;
 #pragma unroll
@@ -385,7 +396,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                 const L *q = S.src + 1 + wr * kLrSw + lane;
                 // u16 samples: |taps| < 2^9 and sample sums < 2^17, 24-bit multiplies
                 constexpr bool k24h = std::is_same<L, uint16_t>::value;
-                auto hm = [](int32_t x, int32_t y, int32_t z) -> int32_t { return k24h ? mad_i24(x, y, z) : x * y + z; };
+                auto hm = [](int32_t x, int32_t y, int32_t z) -> int32_t { return k24h ? mad_i24_s(x, y, z) : x * y + z; };
                 const int32_t acc = hm(hf[0], (int32_t)q[0] + (int32_t)q[6],
                                        hm(hf[1], (int32_t)q[1] + (int32_t)q[5],
                                           hm(hf[2], (int32_t)q[2] + (int32_t)q[4],
@@ -408,20 +419,25 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
             for (int i = 0; i < kLrBand; i++)
                 if (rb + i < b1) {
                     const int32_t acc =
-                        mad_i24(vf[0], h[i] + h[i + 6],
-                                mad_i24(vf[1], h[i + 1] + h[i + 5],
-                                        mad_i24(vf[2], h[i + 2] + h[i + 4], mad_i24(vf[3], h[i + 3], 1 << (r1 - 1)))));
+                        mad_i24_s(vf[0], h[i] + h[i + 6],
+                                  mad_i24_s(vf[1], h[i + 1] + h[i + 5],
+                                            mad_i24_s(vf[2], h[i + 2] + h[i + 4], mad_i24_s(vf[3], h[i + 3], 1 << (r1 - 1)))));
                     sink.put(g.ys + rb + i, g.xs + lane, med3_i32(acc >> r1, 0, maxv));
                 }
         }
         return;
     }
     // self_guided_filter (loop_restoration.rs:485-579) + box_filter (594-729)
+    // The pass radii (r0 in {0, 2}, r1 in {0, 1}, never both 0) are
+    // compile-time in the body below: one instance per radius pair, so the
+    // unrolled row loops carry no per-row radius tests, and n and oneOverN
+    // are constants.
     const uint32_t set = u.sgr_set < 15 ? u.sgr_set : 15;
-    const int32_t r0 = c_sgr_params[set][0], e0 = c_sgr_params[set][1];
-    const int32_t r1 = c_sgr_params[set][2], e1 = c_sgr_params[set][3];
     const int32_t w0 = u.sgr_xqd[0], w1 = u.sgr_xqd[1];
     const int32_t w2 = 128 - w0 - w1;
+    auto body = [&](auto r0c, auto r1c) {
+    constexpr int32_t r0 = decltype(r0c)::value, r1 = decltype(r1c)::value;
+    constexpr int32_t n0 = (2 * r0 + 1) * (2 * r0 + 1), n1 = (2 * r1 + 1) * (2 * r1 + 1);
     // A/B over tile rows -1 .. H and cols -1 .. W: thread -> (column, row
     // band); each walks down its band keeping 3- and 5-wide horizontal sums
     // of the last five window rows, so every box sum costs five LDS reads.
@@ -433,12 +449,12 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
         const uint64_t mc = c_lr_recip[ncol];
         const int nb = (int)lr_mdiv((uint32_t)nt, mc);
         const int band = (int)lr_mdiv((uint32_t)tid, mc), c = tid - band * ncol;
-        const int KR = (int)lr_mdiv((uint32_t)(H + 2 + nb - 1), c_lr_recip[nb]);
+        // rows per band, rounded up to even: every band then starts on an odd
+        // tile row, so a row's parity follows from its unrolled position
+        const int KR = ((int)lr_mdiv((uint32_t)(H + 2 + nb - 1), c_lr_recip[nb]) + 1) & ~1;
         const int k0 = -1 + band * KR, k1 = min(H, k0 + KR - 1);  // inclusive tile rows
         if (band < nb && k0 <= k1) {
-            const LrSgrConsts kc0 = c_sgr_consts[set][0], kc1 = c_sgr_consts[set][1];
-            const int32_t n0 = kc0.n, n1 = kc1.n;
-            const uint32_t s0 = kc0.s, s1 = kc1.s, o0 = kc0.oon, o1 = kc1.oon;
+            const uint32_t s0 = c_sgr_s[set][0], s1 = c_sgr_s[set][1];
             // window row of tile row k is k + 3; column of tile col c - 1 is c + 2
             int32_t h3[5], q3[5], h5[5], q5[5];
             auto hsum = [&](int wr, int slot) {
@@ -462,23 +478,25 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                     const int k = kb + i;
                     if (k > k1) break;
                     hsum(k + 2 + 3, 4);
-                    if (r1) {
+                    if constexpr (r1 != 0) {
                         uint32_t a;
                         int32_t b;
-                        if (r1 == 1) {
+                        if constexpr (r1 == 1) {
                             a = (uint32_t)(q3[1] + q3[2] + q3[3]);
                             b = h3[1] + h3[2] + h3[3];
                         } else {
                             a = (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
                             b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
                         }
-                        S.ab1[(k + 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value>(a, b, n1, s1, o1, (uint32_t)bd, S.a2);
+                        S.ab1[(k + 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value, n1>(a, b, s1, (uint32_t)bd, S.a2);
                     }
-                    if (r0 && ((ys + k) & 1)) {
+                    // (ys + k) & 1 with k = kb + i and kb odd (k0 is -1 plus
+                    // an even number of rows, and steps are even)
+                    if (r0 != 0 && ((ys + 1 + i) & 1)) {
                         const uint32_t a =
                             (uint32_t)q5[0] + (uint32_t)q5[1] + (uint32_t)q5[2] + (uint32_t)q5[3] + (uint32_t)q5[4];
                         const int32_t b = h5[0] + h5[1] + h5[2] + h5[3] + h5[4];
-                        S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value>(a, b, n0, s0, o0, (uint32_t)bd, S.a2);
+                        S.ab0[((k + 1) >> 1) * kLrAw + c] = lr_ab<std::is_same<L, uint16_t>::value, n0>(a, b, s0, (uint32_t)bd, S.a2);
                     }
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
@@ -513,13 +531,13 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
         pb = 5 * ((int32_t)(l & 0xFFFFFu) + (int32_t)(r & 0xFFFFFu)) + 6 * (int32_t)(m & 0xFFFFFu);
     };
     int32_t sa0 = 0, ma0 = 0, sb0 = 0, mb0 = 0, sa1 = 0, ma1 = 0, sb1 = 0, mb1 = 0;
-    if (r1) {
+    if constexpr (r1 != 0) {
         row3(S.ab1, (b0 - 1 + 1) * kLrAw, sa0, ma0, sb0, mb0);
         row3(S.ab1, (b0 + 1) * kLrAw, sa1, ma1, sb1, mb1);
     }
     // pass-0 P of the odd row at or above the current row (tile row -1 is odd)
     int32_t pa_prev = 0, pb_prev = 0;
-    if (r0) {
+    if constexpr (r0 != 0) {
         const int kp = ((ys + b0) & 1) ? b0 : b0 - 1;
         prow(kp, pa_prev, pb_prev);
     }
@@ -528,6 +546,8 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     // int16 xqd), so it runs on the 24-bit multiplier.
     constexpr bool k24 = std::is_same<L, uint16_t>::value;
     auto mad = [](int32_t x, int32_t y, int32_t z) -> int32_t { return k24 ? mad_i24(x, y, z) : x * y + z; };
+    // w0..w2 are the unit's (wave-uniform) weights
+    auto madw = [](int32_t w, int32_t y, int32_t z) -> int32_t { return k24 ? mad_i24_s(w, y, z) : w * y + z; };
     for (int kb = b0; kb < b1; kb += kLrBand) {
 #pragma unroll
         for (int i = 0; i < kLrBand; i++) {
@@ -536,7 +556,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
             const int32_t uval = (int32_t)S.src[1 + (k + 3) * kLrSw + lane + 3];
             const int32_t u4 = uval << 4;
             int32_t f0 = u4, f1 = u4;
-            if (r0) {
+            if constexpr (r0 != 0) {
                 int32_t as, bs;
                 uint32_t sh;
                 if ((ys + k) & 1) {  // P(k) is already held: set at b0 or by row k-1
@@ -556,7 +576,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                 const uint32_t ts = 8 + sh - 4;
                 f0 = (v + (1 << (ts - 1))) >> ts;
             }
-            if (r1) {
+            if constexpr (r1 != 0) {
                 int32_t sa2, ma2, sb2, mb2;
                 row3(S.ab1, (k + 2) * kLrAw, sa2, ma2, sb2, mb2);
                 const int32_t as = 3 * (sa0 + sa2) + ma0 + ma2 + 4 * sa1;
@@ -566,11 +586,22 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                 sa0 = sa1, ma0 = ma1, sb0 = sb1, mb0 = mb1;
                 sa1 = sa2, ma1 = ma2, sb1 = sb2, mb1 = mb2;
             }
-            const int32_t v = mad(w2, r1 != 0 ? f1 : u4, mad(w0, r0 != 0 ? f0 : u4, mad(w1, u4, 1 << 10)));
+            const int32_t v = madw(w2, r1 != 0 ? f1 : u4, madw(w0, r0 != 0 ? f0 : u4, madw(w1, u4, 1 << 10)));
             const int32_t sres = v >> 11;
             sink.sgr(ys + k, g.xs + lane, med3_i32(sres, 0, maxv), uval, f0, f1);
         }
     }
+    };
+    using I0 = std::integral_constant<int32_t, 0>;
+    using I1 = std::integral_constant<int32_t, 1>;
+    using I2 = std::integral_constant<int32_t, 2>;
+    const int32_t r0 = c_sgr_params[set][0], r1 = c_sgr_params[set][2];
+    if (r0 == 0)
+        body(I0{}, I1{});
+    else if (r1 == 0)
+        body(I2{}, I0{});
+    else
+        body(I2{}, I1{});
 }
 
 }  // namespace oxav1
