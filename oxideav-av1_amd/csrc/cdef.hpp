@@ -72,7 +72,7 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
     auto px = [&](int i, int j) -> int32_t {
         if (CLAMP) {
             const int yy = min(y8 + i, lh - 1), xx = min(x8 + j, lw - 1);
-            return (int32_t)t[(yy + kCdefB) * ls + xx + kCdefB] >> shift;
+            return (int32_t)t[mul_u24(yy + kCdefB, ls) + xx + kCdefB] >> shift;
         }
         return (int32_t)base[i * ls + j] >> shift;
     };
@@ -141,26 +141,28 @@ __device__ __forceinline__ void cdef_dir_quarter(const S3<L> *t, int ls, int lh,
     }
 }
 
-template <typename L>
+// FULL: the tile is staged over the whole unit (rows and columns past the
+// plane hold the plane's clamped samples), so no block needs the clamps.
+template <typename L, bool FULL = false>
 __device__ __forceinline__ void cdef_dir_costs(const S3<L> *t, int ls, int lh, int lw, int y8, int x8, int32_t shift,
                                                int quarter, S3<int32_t> *cost) {
-    const bool in = y8 + 8 <= lh && x8 + 8 <= lw;
+    const bool in = FULL || (y8 + 8 <= lh && x8 + 8 <= lw);
     switch (quarter) {  // uniform per wave
     case 0:
         if (in) cdef_dir_quarter<L, 0, false>(t, ls, lh, lw, y8, x8, shift, cost);
-        else cdef_dir_quarter<L, 0, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        else if constexpr (!FULL) cdef_dir_quarter<L, 0, true>(t, ls, lh, lw, y8, x8, shift, cost);
         break;
     case 1:
         if (in) cdef_dir_quarter<L, 1, false>(t, ls, lh, lw, y8, x8, shift, cost);
-        else cdef_dir_quarter<L, 1, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        else if constexpr (!FULL) cdef_dir_quarter<L, 1, true>(t, ls, lh, lw, y8, x8, shift, cost);
         break;
     case 2:
         if (in) cdef_dir_quarter<L, 2, false>(t, ls, lh, lw, y8, x8, shift, cost);
-        else cdef_dir_quarter<L, 2, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        else if constexpr (!FULL) cdef_dir_quarter<L, 2, true>(t, ls, lh, lw, y8, x8, shift, cost);
         break;
     default:
         if (in) cdef_dir_quarter<L, 3, false>(t, ls, lh, lw, y8, x8, shift, cost);
-        else cdef_dir_quarter<L, 3, true>(t, ls, lh, lw, y8, x8, shift, cost);
+        else if constexpr (!FULL) cdef_dir_quarter<L, 3, true>(t, ls, lh, lw, y8, x8, shift, cost);
         break;
     }
 }
@@ -548,8 +550,11 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
     const int32_t oy0 = (int32_t)r0 * 4, ox0 = (int32_t)c0 * 4;
     const int tw0 = min(kCdefUnit, ew0 - ox0), th0 = min(kCdefUnit, eh0 - oy0);
     if (tw0 <= 0 || th0 <= 0) return;
-    // staged extent: the unit inside the buffer (>= the walked extent)
-    const int lw0 = min(kCdefUnit, pw0 - ox0), lh0 = min(kCdefUnit, ph0 - oy0);
+    // staged extent: the whole unit, coordinates clamped to the buffer, so
+    // the direction search of blocks that reach past the buffer reads the
+    // clamped samples (min(y, lh - 1), min(x, lw - 1) with lh, lw the unit's
+    // extent inside the buffer) without clamps of its own
+    const int lw0 = kCdefUnit, lh0 = kCdefUnit;
     const int ls0 = lw0 + 2 * kCdefB;
     L *t0 = lds;
     cdef_load<L, T, kCdefPacked<T>>(t0, lw0, lh0, ox0, oy0, pw0, ph0, plane_ptr<T>(J.deb[0]), J.deb_pitch[0]);
@@ -574,7 +579,7 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
     {
         const int b = tid & 63, quarter = tid >> 6;  // a wave per pair of directions
         if (quarter < 4 && block_active(b))
-            cdef_dir_costs<L>((const S3<L> *)t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, quarter,
+            cdef_dir_costs<L, true>((const S3<L> *)t0, ls0, lh0, lw0, 8 * (b >> 3), 8 * (b & 7), coeff_shift, quarter,
                               (S3<int32_t> *)(costs + b * 8));
     }
     __syncthreads();
