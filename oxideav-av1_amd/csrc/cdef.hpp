@@ -21,6 +21,7 @@ constexpr int kCdefLw = kCdefUnit + 2 * kCdefB;  // 68
 // element of the first copy starts at an even one of the second: every pair
 // the packed filter reads is an aligned dword.
 constexpr int kCdefDual = kCdefLw * kCdefLw + 1;
+typedef uint32_t cdef_u32x2 __attribute__((ext_vector_type(2)));
 
 __constant__ int8_t c_cdef_dir[8][2][2] = {
     {{-1, 1}, {-2, 2}}, {{0, 1}, {-1, 2}}, {{0, 1}, {0, 2}}, {{0, 1}, {1, 2}},
@@ -458,6 +459,79 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
     const int ng = (ls + 5) >> 2;
     const uint32_t m = ((1u << 20) + (uint32_t)ng - 1) / (uint32_t)ng;
     const int n = ng * lh;
+    if constexpr (sizeof(L) == 2 && sizeof(T) <= 2) {
+        // u8/u16 samples: thread t loads group column t % ng of rows t / ng,
+        // t / ng + rpp, ... (rpp = rows per pass), up to five rows at once
+        // (a whole 68-row unit), one aligned vector load per group, always
+        // inside its row (a group past either plane edge loads the nearest
+        // group inside, ga, and picks its clamped samples from it at the
+        // store); the samples stay packed in registers until the LDS stores
+        constexpr int kR = 5;
+        using W = typename std::conditional<sizeof(T) == 1, uint32_t, cdef_u32x2>::type;
+        const int tid = (int)threadIdx.x;
+        const int rpp = div_magic((int)blockDim.x, m);  // m = ceil(2^20 / ng)
+        const int r0 = div_magic(tid, m), g = tid - (int)mul_u24(r0, ng);
+        if (r0 >= rpp) return;
+        const int32_t gx = ox - 4 + 4 * g;
+        const int32_t ga = clip3(0, (pw - 1) & ~3, gx);  // the last group holding a plane column at most
+        const int x0 = 4 * g - kCdefB;
+        // sample k is plane column clip3(0, pw - 1, gx + k), at position
+        // j_k = that - ga (0..3) of the loaded group
+        const bool edge = gx < 0 || gx + 3 >= pw;
+        for (int yb = r0; yb < lh; yb += kR * rpp) {
+            W w[kR];
+#pragma unroll
+            for (int k = 0; k < kR; k++) {
+                const int yy = yb + k * rpp;
+                if (yy < lh) w[k] = *(const G<W> *)(src + (size_t)clip3(0, ph - 1, oy + yy - kCdefB) * spitch + ga);
+            }
+#pragma unroll
+            for (int k = 0; k < kR; k++) {
+                const int yy = yb + k * rpp;
+                if (yy >= lh) continue;
+                const int e = (int)mul_u24(yy, ls) + x0;
+                // samples 0..3 of the group as (0, 1) and (2, 3) pairs
+                uint32_t lo, hi;
+                if constexpr (sizeof(T) == 1) {
+                    lo = (w[k] & 0xffu) | ((w[k] << 8) & 0xff0000u);
+                    hi = ((w[k] >> 16) & 0xffu) | ((w[k] >> 8) & 0xff0000u);
+                } else {
+                    lo = w[k].x;
+                    hi = w[k].y;
+                }
+                if (edge) {
+                    uint32_t s4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int32_t j = clip3(0, pw - 1, gx + q) - ga;
+                        s4[q] = ((j < 2 ? lo : hi) >> (16 * (j & 1))) & 0xffffu;
+                    }
+                    lo = s4[0] | s4[1] << 16;
+                    hi = s4[2] | s4[3] << 16;
+                }
+                if (x0 >= 0 && x0 + 3 < ls && !(e & 1)) {
+                    uint32_t *d = (uint32_t *)(t + e);
+                    d[0] = lo;
+                    d[1] = hi;
+                    if (DUAL) {  // kCdefDual is odd: e + kCdefDual + 1 is even
+                        L *q = t + kCdefDual + e;
+                        q[0] = (L)lo;
+                        *(uint32_t *)(q + 1) = __builtin_amdgcn_alignbit(hi, lo, 16);
+                        q[3] = (L)(hi >> 16);
+                    }
+                    continue;
+                }
+                const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (x0 + q >= 0 && x0 + q < ls) {
+                        t[e + q] = (L)s4[q];
+                        if (DUAL) t[kCdefDual + e + q] = (L)s4[q];
+                    }
+            }
+        }
+        return;
+    }
     constexpr int kR = 2;  // groups per thread and round (more raise the kernel's register use)
     for (int i0 = 0; i0 < n; i0 += kR * (int)blockDim.x) {
         int32_t v[kR][4];
@@ -473,25 +547,12 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
                 const G<T> *row = src + (size_t)clip3(0, ph - 1, oy + yy - kCdefB) * spitch;
                 const int32_t gx = ox - 4 + 4 * g;
                 if (gx >= 0 && gx + 3 < pw) {
-                    if constexpr (sizeof(T) == 1) {
-                        const uint32_t w = *(const G<uint32_t> *)(row + gx);
-#pragma unroll
-                        for (int k = 0; k < 4; k++) v[r][k] = (int32_t)((w >> (8 * k)) & 255u);
-                    } else if constexpr (sizeof(T) == 2) {
-                        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-                        const u32x2 w = *(const G<u32x2> *)(row + gx);
-                        v[r][0] = (int32_t)(w.x & 0xffffu);
-                        v[r][1] = (int32_t)(w.x >> 16);
-                        v[r][2] = (int32_t)(w.y & 0xffffu);
-                        v[r][3] = (int32_t)(w.y >> 16);
-                    } else {
-                        typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
-                        const i32x4 w = *(const G<i32x4> *)(row + gx);
-                        v[r][0] = w.x;
-                        v[r][1] = w.y;
-                        v[r][2] = w.z;
-                        v[r][3] = w.w;
-                    }
+                    typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+                    const i32x4 w = *(const G<i32x4> *)(row + gx);
+                    v[r][0] = w.x;
+                    v[r][1] = w.y;
+                    v[r][2] = w.z;
+                    v[r][3] = w.w;
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) v[r][k] = (int32_t)row[clip3(0, pw - 1, gx + k)];
@@ -502,20 +563,6 @@ __device__ __forceinline__ void cdef_load(L *t, int tw, int th, int32_t ox, int3
         for (int r = 0; r < kR; r++) {
             if (yr[r] < 0) continue;
             const int x0 = 4 * gr[r] - kCdefB, e = (int)mul_u24(yr[r], ls) + x0;
-            if constexpr (sizeof(L) == 2) {
-                if (x0 >= 0 && x0 + 3 < ls && !(e & 1)) {
-                    uint32_t *d = (uint32_t *)(t + e);
-                    d[0] = (uint32_t)v[r][0] | (uint32_t)v[r][1] << 16;
-                    d[1] = (uint32_t)v[r][2] | (uint32_t)v[r][3] << 16;
-                    if (DUAL) {  // kCdefDual is odd: e + kCdefDual + 1 is even
-                        L *q = t + kCdefDual + e;
-                        q[0] = (L)v[r][0];
-                        *(uint32_t *)(q + 1) = (uint32_t)v[r][1] | (uint32_t)v[r][2] << 16;
-                        q[3] = (L)v[r][3];
-                    }
-                    continue;
-                }
-            }
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (x0 + k >= 0 && x0 + k < ls) {
