@@ -348,7 +348,14 @@ union CdefTables {
     CdefTapsPk pk[64];
 };
 
-template <typename T>
+// PRI / SEC: some block of the plane has a nonzero primary / secondary
+// strength (both are per unit, up to the per-block primary adjustment to 0).
+// A tap whose strength is 0 adds 0 (constrain returns 0), and with only
+// primary or only secondary taps the weights sum to 12 < 16, so
+// x + ((8 + sum - (sum < 0)) >> 4) stays between the smallest and the
+// largest tap (x included) and the final clip is the identity: the
+// instances without one kind skip its taps and the min/max tracking.
+template <typename T, bool PRI = true, bool SEC = true>
 __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, int tw, int th, int32_t ox, int32_t oy,
                                                      int32_t bw_log2, int32_t bh_log2, const CdefTapsPk *taps,
                                                      int32_t row_log2, G<T> *__restrict__ dst, uint32_t dpitch) {
@@ -379,6 +386,7 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
             s16x2 mx = x, mn = x;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
+                if ((i < 2 && !PRI) || (i >= 2 && !SEC)) continue;
                 const u16x2 thr = i < 2 ? thr_p : thr_s;
                 const u16x2 adj = i < 2 ? adj_p : adj_s;
 #pragma unroll
@@ -395,8 +403,10 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
                         acc[2] += cv;
                     else
                         sec1 += cv;
-                    mx = __builtin_elementwise_max(mx, v);
-                    mn = __builtin_elementwise_min(mn, v);
+                    if (PRI && SEC) {
+                        mx = __builtin_elementwise_max(mx, v);
+                        mn = __builtin_elementwise_min(mn, v);
+                    }
                 }
             }
             const s16x2 w0 = __builtin_bit_cast(s16x2, splat(b.pt0 & 0x7fu));
@@ -405,7 +415,7 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
             // (8 + sum - (sum < 0)) >> 4, arithmetic shifts on each half
             const s16x2 neg = sum >> (s16x2){15, 15};
             const s16x2 delta = (sum + neg + (s16x2){8, 8}) >> (s16x2){4, 4};
-            res = __builtin_elementwise_min(__builtin_elementwise_max(x + delta, mn), mx);
+            res = (PRI && SEC) ? __builtin_elementwise_min(__builtin_elementwise_max(x + delta, mn), mx) : x + delta;
         }
         const int32_t py = oy + yy, px = ox + xx;
         const uint32_t r = __builtin_bit_cast(uint32_t, res);
@@ -415,6 +425,19 @@ __device__ __forceinline__ void cdef_filter_plane_pk(const uint16_t *t, int ls, 
             *(G<uint16_t> *)(dst + (size_t)py * dpitch + px) = (uint16_t)((r & 0xffu) | ((r >> 8) & 0xff00u));
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void cdef_filter_plane_pk_s(bool pri, bool sec, const uint16_t *t, int ls, int tw, int th,
+                                                       int32_t ox, int32_t oy, int32_t bw_log2, int32_t bh_log2,
+                                                       const CdefTapsPk *taps, int32_t row_log2, G<T> *__restrict__ dst,
+                                                       uint32_t dpitch) {
+    if (pri && sec)
+        cdef_filter_plane_pk<T, true, true>(t, ls, tw, th, ox, oy, bw_log2, bh_log2, taps, row_log2, dst, dpitch);
+    else if (pri)
+        cdef_filter_plane_pk<T, true, false>(t, ls, tw, th, ox, oy, bw_log2, bh_log2, taps, row_log2, dst, dpitch);
+    else
+        cdef_filter_plane_pk<T, false, true>(t, ls, tw, th, ox, oy, bw_log2, bh_log2, taps, row_log2, dst, dpitch);
 }
 
 template <typename L, typename T, bool CHECK>
@@ -668,7 +691,9 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
         G<T> *dst = plane_ptr<T>(J.cdf[0]);
         if constexpr (kCdefPacked<T>)
             if (packed) {
-                cdef_filter_plane_pk<T>(t0, ls0, tw0, th0, ox0, oy0, 3, 3, tabs->pk, 6, dst, J.cdf_pitch[0]);
+                const bool pri = idx >= 0 && J.cdef.y_pri_strength[idx] != 0;
+                const bool sec = idx >= 0 && J.cdef.y_sec_strength[idx] != 0;
+                cdef_filter_plane_pk_s<T>(pri, sec, t0, ls0, tw0, th0, ox0, oy0, 3, 3, tabs->pk, 6, dst, J.cdf_pitch[0]);
                 goto chroma;
             }
         if (inside)
@@ -706,8 +731,10 @@ chroma:
         G<T> *dst = plane_ptr<T>(J.cdf[p]);
         if constexpr (kCdefPacked<T>)
             if (packed) {
-                cdef_filter_plane_pk<T>(t0, ls, tw, th, ox, oy, 3 - sx, 3 - sy, tabs->pk, 6 - sx, dst,
-                                        J.cdf_pitch[p]);
+                const bool pri = idx >= 0 && J.cdef.uv_pri_strength[idx] != 0;
+                const bool sec = idx >= 0 && J.cdef.uv_sec_strength[idx] != 0;
+                cdef_filter_plane_pk_s<T>(pri, sec, t0, ls, tw, th, ox, oy, 3 - sx, 3 - sy, tabs->pk, 6 - sx, dst,
+                                          J.cdf_pitch[p]);
                 continue;
             }
         if (inside)
