@@ -124,6 +124,19 @@ __device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
     return r;
 }
 
+// The same with wave-uniform bounds: hi from a scalar register (one scalar
+// operand is all a VOP3 instruction reads), lo 0 as an inline constant.
+__device__ __forceinline__ int32_t med3_i32_s(int32_t v, int32_t lo, int32_t hi) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "s"(hi));
+    return r;
+}
+__device__ __forceinline__ int32_t clamp0_s(int32_t v, int32_t hi) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "s"(hi));
+    return r;
+}
+
 // Products of operands that fit 24 bits (two's complement for the i24 forms,
 // unsigned for the u24 forms) as single full-rate instructions: the low 32
 // bits of x * y (+ z).  The compiler picks these by itself only where it can

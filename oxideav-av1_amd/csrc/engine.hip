@@ -629,6 +629,9 @@ static int lr_set_magic(FrameJob &J) {
         J.lr_magic_us[p] = lr_magic(J.lr_unit_size[p]);
     }
     if (J.item_base[kStageLr][3] >= (1u << 24)) return OXAV1_ERR_UNSUPPORTED;
+    // the loop-restoration output addresses rows with 32-bit offsets
+    for (uint32_t p = 0; p < 3; p++)
+        if ((uint64_t)J.out_rows[p] * J.ref_pitch[p] >= (1ull << 31)) return OXAV1_ERR_UNSUPPORTED;
     return OXAV1_OK;
 }
 

@@ -38,6 +38,7 @@ constexpr int kLrAh0 = kLrMaxH / 2 + 1;     // pass-0 A/B rows: the odd rows onl
 // workgroup: the row loops are unrolled to these bounds, so their register
 // windows need no rotating copies.
 constexpr int kLrBand = (kLrMaxH + 3) / 4;
+static_assert(kLrBand % 2 == 0, "output row steps keep the row parity");
 constexpr int kLrAbBand = (kLrMaxH + 2 + 2) / 3;
 static_assert(kLrAbBand % 2 == 0, "A/B row steps keep the row parity");
 
@@ -177,13 +178,17 @@ struct LrStore {
     __device__ __forceinline__ bool copy4_ok() const {
         return (opitch & 3u) == 0 && ((uintptr_t)out & (4 * sizeof(T) - 1)) == 0;
     }
+    // offsets in 32 bits: rows x pitch of an output plane stays below 2^31
+    // (checked when the job is built)
     __device__ __forceinline__ void copy4(int32_t y, int32_t x, typename LrVec<T>::type v) {
-        *(G<typename LrVec<T>::type> *)(out + (size_t)y * opitch + x) = v;
+        *(G<typename LrVec<T>::type> *)(out + ((uint32_t)y * opitch + (uint32_t)x)) = v;
     }
     __device__ __forceinline__ void unit(uint32_t, uint32_t, const oxav1_lr_unit &) {}
-    __device__ __forceinline__ void put(int32_t y, int32_t x, int32_t v) { out[(size_t)y * opitch + x] = (T)v; }
+    __device__ __forceinline__ void put(int32_t y, int32_t x, int32_t v) {
+        out[(uint32_t)y * opitch + (uint32_t)x] = (T)v;
+    }
     __device__ __forceinline__ void sgr(int32_t y, int32_t x, int32_t v, int32_t, int32_t, int32_t) {
-        out[(size_t)y * opitch + x] = (T)v;
+        out[(uint32_t)y * opitch + (uint32_t)x] = (T)v;
     }
 };
 
@@ -254,7 +259,9 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     if (g.ys > g.ye) return;
     const int32_t H = g.ye - g.ys + 1;
     // Output rows are split into one band per wave; a lane owns a column.
-    const int RB = (H + nwv - 1) >> 2;
+    // (an even count: the first rows ys and b0 are then even, and so is
+    // every row step, so a row's parity is its unrolled position's)
+    const int RB = (((H + nwv - 1) >> 2) + 1) & ~1;
     const int b0 = wv * RB, b1 = min(H, b0 + RB);
     using V = typename LrVec<T>::type;
     if (type != OXAV1_RESTORE_WIENER && type != OXAV1_RESTORE_SGRPROJ) {
@@ -401,7 +408,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                                        hm(hf[1], (int32_t)q[1] + (int32_t)q[5],
                                           hm(hf[2], (int32_t)q[2] + (int32_t)q[4],
                                              hm(hf[3], (int32_t)q[3], 1 << (r0 - 1)))));
-                S.inter[wr * kLrMaxW + lane] = med3_i32(acc >> r0, -offset, limit - offset);
+                S.inter[wr * kLrMaxW + lane] = med3_i32_s(acc >> r0, -offset, limit - offset);
             }
         }
         __syncthreads();
@@ -422,7 +429,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
                         mad_i24_s(vf[0], h[i] + h[i + 6],
                                   mad_i24_s(vf[1], h[i + 1] + h[i + 5],
                                             mad_i24_s(vf[2], h[i + 2] + h[i + 4], mad_i24_s(vf[3], h[i + 3], 1 << (r1 - 1)))));
-                    sink.put(g.ys + rb + i, g.xs + lane, med3_i32(acc >> r1, 0, maxv));
+                    sink.put(g.ys + rb + i, g.xs + lane, clamp0_s(acc >> r1, maxv));
                 }
         }
         return;
@@ -559,7 +566,8 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
             if constexpr (r0 != 0) {
                 int32_t as, bs;
                 uint32_t sh;
-                if ((ys + k) & 1) {  // P(k) is already held: set at b0 or by row k-1
+                // (ys + k) & 1: ys, kb and the steps are even
+                if (i & 1) {  // P(k) is already held: set at b0 or by row k-1
                     as = pa_prev;
                     bs = pb_prev;
                     sh = 4;
@@ -588,7 +596,7 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
             }
             const int32_t v = madw(w2, r1 != 0 ? f1 : u4, madw(w0, r0 != 0 ? f0 : u4, madw(w1, u4, 1 << 10)));
             const int32_t sres = v >> 11;
-            sink.sgr(ys + k, g.xs + lane, med3_i32(sres, 0, maxv), uval, f0, f1);
+            sink.sgr(ys + k, g.xs + lane, clamp0_s(sres, maxv), uval, f0, f1);
         }
     }
     };
