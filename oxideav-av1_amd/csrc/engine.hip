@@ -59,6 +59,17 @@ __device__ __forceinline__ ItemRef find_item(const uint32_t *prefix, uint32_t nj
     }
     return {lo, item - prefix[lo]};
 }
+// The job record in the constant address space: its fields become scalar
+// loads (s_load through the scalar cache).  Only for reads at 4-byte aligned
+// addresses: a byte array indexed by a run-time value would give the scalar
+// loads a misaligned base.
+using JobC = const __attribute__((address_space(4))) FrameJob;
+__device__ __forceinline__ JobC &job_c(const FrameJob &J) { return *(JobC *)&J; }
+
+// plane of item `local` among three planes' item ranges starting at base[0..2]
+__device__ __forceinline__ uint32_t plane_of3(const __attribute__((address_space(4))) uint32_t *base, uint32_t local) {
+    return local >= base[2] ? 2u : (local >= base[1] ? 1u : 0u);
+}
 __device__ __forceinline__ uint32_t plane_of(const uint32_t *base, uint32_t nplanes, uint32_t local) {
     uint32_t p = 0;
     while (p + 1 < nplanes && local >= base[p + 1]) p++;
@@ -114,17 +125,17 @@ __device__ __forceinline__ void sr_item(const FrameJob &J, uint32_t local, const
 
 template <typename T>
 __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSmem<typename LdsOf<T>::type> &S) {
-    const uint32_t p = plane_of(J.item_base[kStageLr], 3, local);
-    const uint32_t t = local - J.item_base[kStageLr][p];
     // the job record is read-only for the whole launch: through the constant
-    // address space its fields are scalar loads (every base 4-byte aligned:
-    // lr_tile reads no byte array indexed by the plane)
-    using JC = const __attribute__((address_space(4))) FrameJob;
-    JC &JK = *(JC *)&J;
+    // address space its fields are scalar loads through the scalar cache
+    // instead of vector loads (every base 4-byte aligned: nothing here or in
+    // lr_tile reads a byte array indexed by the plane)
+    JobC &JK = job_c(J);
+    const uint32_t p = plane_of3(JK.item_base[kStageLr], local);
+    const uint32_t t = local - JK.item_base[kStageLr][p];
     uint32_t stripe, chunk, tile;
     lr_decode(JK, p, t, stripe, chunk, tile);
-    LrStore<T> sink{plane_ptr<T>(J.ref_out[p]), J.ref_pitch[p]};
-    lr_tile<T, LrStore<T>, JC>(JK, p, stripe, chunk, tile, S, sink);
+    LrStore<T> sink{plane_ptr<T>(JK.ref_out[p]), JK.ref_pitch[p]};
+    lr_tile<T, LrStore<T>, JobC>(JK, p, stripe, chunk, tile, S, sink);
 }
 
 template <typename T>
@@ -289,7 +300,7 @@ __device__ __forceinline__ void lr_body(const FrameJob *jobs, const uint32_t *pr
         if constexpr (KIND == 1) {
             lr_item<int32_t>(J, ref.local, S);
         } else {
-            if (J.kind == kU8)
+            if (job_c(J).kind == kU8)
                 lr_item<uint8_t>(J, ref.local, S);
             else
                 lr_item<uint16_t>(J, ref.local, S);
