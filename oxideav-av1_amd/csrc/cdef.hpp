@@ -674,8 +674,9 @@ __device__ __forceinline__ void cdef_unit(const FrameJob &J, uint32_t ur, uint32
         }
         blk[tid] = b;
     }
-    __syncthreads();
-    // luma filter
+    // luma filter (the taps of block tid are resolved by thread tid, which
+    // wrote blk[tid]: no barrier between; tabs->costs, which the taps
+    // overwrite, were read by the same wave before)
     {
         const int32_t damping = (int32_t)J.cdef.cdef_damping + coeff_shift;
         const bool inside = ox0 >= kCdefB && oy0 >= kCdefB && ox0 + tw0 + kCdefB <= pw0 && oy0 + th0 + kCdefB <= ph0;
