@@ -79,17 +79,19 @@ __device__ __forceinline__ uint32_t plane_of(const uint32_t *base, uint32_t npla
 // --- stage bodies (templated on the sample type) ---
 template <typename T>
 __device__ __forceinline__ void lf_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *region, LfShared &sh) {
-    const uint32_t p = plane_of(J.item_base[kStageLf], 3, local);
-    const uint32_t t = local - J.item_base[kStageLf][p];
-    const uint32_t ty = t / J.lf_tiles_x[p], tx = t - ty * J.lf_tiles_x[p];
-    lf_tile<T>(J, p, ty, tx, plane_ptr<T>(J.src[p]), J.src_pitch[p], plane_ptr<T>(J.deb[p]), J.deb_pitch[p], region,
+    JobC &JK = job_c(J);
+    const uint32_t p = plane_of3(JK.item_base[kStageLf], local);
+    const uint32_t t = local - JK.item_base[kStageLf][p];
+    const uint32_t ty = t / JK.lf_tiles_x[p], tx = t - ty * JK.lf_tiles_x[p];
+    lf_tile<T>(J, p, ty, tx, plane_ptr<T>(JK.src[p]), JK.src_pitch[p], plane_ptr<T>(JK.deb[p]), JK.deb_pitch[p], region,
                sh);
 }
 
 template <typename T>
 __device__ __forceinline__ void cdef_item(const FrameJob &J, uint32_t local, typename LdsOf<T>::type *tile, CdefBlk *blk,
                           CdefTables *tabs) {
-    const uint32_t ur = local / J.cdef_ucols, uc = local - ur * J.cdef_ucols;
+    const uint32_t ucols = job_c(J).cdef_ucols;
+    const uint32_t ur = local / ucols, uc = local - ur * ucols;
     cdef_unit<T>(J, ur, uc, tile, blk, tabs);
 }
 
@@ -140,23 +142,26 @@ __device__ __forceinline__ void lr_item(const FrameJob &J, uint32_t local, LrSme
 
 template <typename T>
 __device__ __forceinline__ void fg_item(const FrameJob &J, uint32_t local, uint8_t *lut, const void *&lut_of) {
-    const uint32_t p = plane_of(J.item_base[kStageFg], 3, local);
-    const uint32_t t = local - J.item_base[kStageFg][p];
-    const uint32_t ty = t / J.fg_tiles_x[p], tx = t - ty * J.fg_tiles_x[p];
-    // the frame's scaling LUTs, staged again only when the frame changes
-    // (jobs without grain have no grain scratch and only copy)
-    if (J.apply_fg && J.fgs != lut_of) {
-        for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)J.fgs)->lut[i >> 8][i & 255];
-        lut_of = J.fgs;
-    }
-    __syncthreads();
-    const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
-    const int32_t y1 = min(y0 + 64, (int32_t)J.out_rows[p]), x1 = min(x0 + 64, (int32_t)J.out_cols[p]);
     // the job record is read-only for the whole launch: through the constant
-    // address space its fields are scalar loads (fg_blend_tile indexes no
-    // byte array of it by the plane)
-    using JC = const __attribute__((address_space(4))) FrameJob;
-    fg_blend_tile<T, JC>(*(JC *)&J, p, y0, y1, x0, x1, lut);
+    // address space its fields are scalar loads (nothing here or in
+    // fg_blend_tile indexes a byte array of it by the plane)
+    JobC &JK = job_c(J);
+    const uint32_t p = plane_of3(JK.item_base[kStageFg], local);
+    const uint32_t t = local - JK.item_base[kStageFg][p];
+    const uint32_t ty = t / JK.fg_tiles_x[p], tx = t - ty * JK.fg_tiles_x[p];
+    // the frame's scaling LUTs, staged again only when the frame changes
+    // (jobs without grain have no grain scratch and only copy); the item
+    // loop's trailing barrier has ordered the last item's LUT reads, and the
+    // condition is uniform in the workgroup, so the barrier after the
+    // staging runs only when there was one
+    if (JK.apply_fg && JK.fgs != lut_of) {
+        for (int i = threadIdx.x; i < 768; i += blockDim.x) lut[i] = ((const G<FgScratch> *)JK.fgs)->lut[i >> 8][i & 255];
+        lut_of = JK.fgs;
+        __syncthreads();
+    }
+    const int32_t y0 = (int32_t)ty * 64, x0 = (int32_t)tx * 64;
+    const int32_t y1 = min(y0 + 64, (int32_t)JK.out_rows[p]), x1 = min(x0 + 64, (int32_t)JK.out_cols[p]);
+    fg_blend_tile<T, JobC>(JK, p, y0, y1, x0, x1, lut);
 }
 
 // --- kernels: KIND 0 = engine (u8/u16 by FrameJob::kind), 1 = int32 ---
@@ -244,7 +249,7 @@ __device__ __forceinline__ void lf_body(const FrameJob *jobs, const uint32_t *pr
         if constexpr (KIND == 1) {
             lf_item<int32_t>(J, ref.local, region, sh);
         } else {
-            if (J.kind == kU8)
+            if (job_c(J).kind == kU8)
                 lf_item<uint8_t>(J, ref.local, region, sh);
             else
                 lf_item<uint16_t>(J, ref.local, region, sh);
@@ -264,7 +269,7 @@ __device__ __forceinline__ void cdef_body(const FrameJob *jobs, const uint32_t *
         if constexpr (KIND == 1) {
             cdef_item<int32_t>(J, ref.local, tile, blk, &tabs);
         } else {
-            if (J.kind == kU8)
+            if (job_c(J).kind == kU8)
                 cdef_item<uint8_t>(J, ref.local, tile, blk, &tabs);
             else
                 cdef_item<uint16_t>(J, ref.local, tile, blk, &tabs);
@@ -283,7 +288,7 @@ __device__ __forceinline__ void sr_body(const FrameJob *jobs, const uint32_t *pr
         if constexpr (KIND == 1) {
             sr_item<int32_t>(J, ref.local, ftab);
         } else {
-            if (J.kind == kU8)
+            if (job_c(J).kind == kU8)
                 sr_item<uint8_t>(J, ref.local, ftab);
             else
                 sr_item<uint16_t>(J, ref.local, ftab);
@@ -317,7 +322,7 @@ __device__ __forceinline__ void fg_body(const FrameJob *jobs, const uint32_t *pr
         if constexpr (KIND == 1) {
             fg_item<int32_t>(J, ref.local, lut, lut_of);
         } else {
-            if (J.kind == kU8)
+            if (job_c(J).kind == kU8)
                 fg_item<uint8_t>(J, ref.local, lut, lut_of);
             else
                 fg_item<uint16_t>(J, ref.local, lut, lut_of);
