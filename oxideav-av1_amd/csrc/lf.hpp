@@ -315,7 +315,9 @@ __device__ __forceinline__ void lf_filter_sample(const LfRegion<L, EDGE> &R, int
         const int32_t half = 0x80 << bds;
         const int32_t ps1 = p[1] - half, ps0 = p[0] - half, qs0 = q[0] - half, qs1 = q[1] - half;
         int32_t filt = hev ? f4clamp(ps1 - qs1, lo, hi) : 0;
-        filt = f4clamp(filt + 3 * (qs0 - ps0), lo, hi);
+        // 16-bit samples: |qs0 - ps0| < 2^16, a 24-bit product
+        const int32_t d3 = sizeof(L) == 2 ? mul_i24k<3>(qs0 - ps0) : 3 * (qs0 - ps0);
+        filt = f4clamp(filt + d3, lo, hi);
         const int32_t f1 = f4clamp(filt + 4, lo, hi) >> 3;
         const int32_t f2 = f4clamp(filt + 3, lo, hi) >> 3;
         R.set(y, x, f4clamp(qs0 - f1, lo, hi) + half);

@@ -67,6 +67,14 @@ __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ip
             v[r] = (int32_t)*(const G<T> *)(row + lc);
         }
     };
+    // sample x tap + sum: u8/u16 samples are below 2^16 (said to the
+    // compiler, which then forms full-rate 24-bit multiply-adds with the
+    // filter's int16 taps instead of 32-bit multiplies); int32 samples keep
+    // 32-bit multiplies
+    auto tap = [](int32_t v, int32_t f, int32_t acc) -> int32_t {
+        if constexpr (sizeof(T) <= 2) __builtin_assume((uint32_t)v < 65536u);
+        return v * f + acc;
+    };
     int32_t px, ph, base, top;
     seg(xb, px, ph, base, top);
     int32_t v[4] = {0, 0, 0, 0};
@@ -100,10 +108,10 @@ __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ip
                     if (y + r >= yb) break;
                     int32_t sum = 0;
 #pragma unroll
-                    for (int k = 0; k < 8; k++) sum += __builtin_amdgcn_ds_bpermute(sel[k], v[r]) * f[k];
+                    for (int k = 0; k < 8; k++) sum = tap(__builtin_amdgcn_ds_bpermute(sel[k], v[r]), f[k], sum);
                     if (store) {
                         G<uint8_t> *orow = (G<uint8_t> *)(out + (size_t)(y + r) * opitch);
-                        *(G<T> *)(orow + (uint32_t)x * (uint32_t)sizeof(T)) = (T)clip3(0, maxv, (sum + 64) >> 7);
+                        *(G<T> *)(orow + (uint32_t)x * (uint32_t)sizeof(T)) = (T)clamp0_s((sum + 64) >> 7, maxv);
                     }
                 }
 #pragma unroll
@@ -114,8 +122,8 @@ __device__ __forceinline__ void sr_rows(const G<T> *__restrict__ in, uint32_t ip
                 const G<T> *row = in + (size_t)y * ipitch;
                 int32_t sum = 0;
 #pragma unroll
-                for (int k = 0; k < 8; k++) sum += (int32_t)row[col[k]] * f[k];
-                if (store) out[(size_t)y * opitch + x] = (T)clip3(0, maxv, (sum + 64) >> 7);
+                for (int k = 0; k < 8; k++) sum = tap((int32_t)row[col[k]], f[k], sum);
+                if (store) out[(size_t)y * opitch + x] = (T)clamp0_s((sum + 64) >> 7, maxv);
             }
             if (fast_next) load4(basen, ya, v);
         }
