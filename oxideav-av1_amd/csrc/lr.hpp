@@ -101,7 +101,7 @@ __device__ __forceinline__ uint32_t lr_ab(uint32_t a, int32_t b, uint32_t s, uin
     // every depth, rounding included), and s <= 3236 for r = 1, s <= 140 for
     // r = 2: p * s < 2^32 (4.25e9 at most), as the reference's saturating
     // 64-bit product never saturates.  (256 - a2) * oneOverN <= 255 * 455
-    // and b < 2^17, so b2 < 2^32.  K24: ar < 2^21, 0 <= dr <= 25 * 255 + 1
+    // and b < 2^17, so b2 < 2^32.  K24: ar < 2^21, 0 <= dr <= 25 * 256
     // and p < 2^24, so every product runs on the 24-bit multiplier.
     uint32_t a2, b2;
     if constexpr (K24) {
