@@ -55,8 +55,12 @@ __constant__ int32_t c_sgr_params[16][4];   // filled from kSgrParams at init
 __constant__ uint16_t c_sgr_a2[256];        // a2 of z = 0 .. 255 (loop_restoration.rs:648-654)
 
 
-// A (<= 256, 9 bits) and B (< 2^20) of one self-guided position in one word.
-__device__ __forceinline__ uint32_t lr_pack_ab(uint32_t a, uint32_t b) { return (a << 20) | b; }
+// A (<= 256, 9 bits) at bit 22 and B (< 2^20) below it, of one self-guided
+// position in one word.  Two or three words add field by field without a
+// carry between the fields (B sums < 3 * 2^20 < 2^22, A sums <= 768 < 2^10),
+// so neighbouring positions are summed before the fields are split.
+constexpr uint32_t kLrAbShift = 22, kLrAbMask = (1u << kLrAbShift) - 1;
+__device__ __forceinline__ uint32_t lr_pack_ab(uint32_t a, uint32_t b) { return (a << kLrAbShift) | b; }
 
 // Chunks per stripe: a luma stripe (64 rows) has two, a vertically
 // subsampled chroma stripe (32 rows) one.
@@ -526,16 +530,18 @@ __device__ __forceinline__ void lr_tile(JT &J, uint32_t plane, uint32_t stripe, 
     const int32_t ys = g.ys;
     auto row3 = [&](const uint32_t *ab, int idx, int32_t &sa, int32_t &ma, int32_t &sb, int32_t &mb) {
         const uint32_t l = ab[idx + lane], m = ab[idx + lane + 1], r = ab[idx + lane + 2];
-        ma = (int32_t)(m >> 20);
-        mb = (int32_t)(m & 0xFFFFFu);
-        sa = (int32_t)(l >> 20) + ma + (int32_t)(r >> 20);
-        sb = (int32_t)(l & 0xFFFFFu) + mb + (int32_t)(r & 0xFFFFFu);
+        const uint32_t s3 = l + m + r;  // field by field (lr_pack_ab)
+        ma = (int32_t)(m >> kLrAbShift);
+        mb = (int32_t)(m & kLrAbMask);
+        sa = (int32_t)(s3 >> kLrAbShift);
+        sb = (int32_t)(s3 & kLrAbMask);
     };
     auto prow = [&](int k, int32_t &pa, int32_t &pb) {  // P at odd tile row k
         const int idx = ((k + 1) >> 1) * kLrAw;
         const uint32_t l = S.ab0[idx + lane], m = S.ab0[idx + lane + 1], r = S.ab0[idx + lane + 2];
-        pa = 5 * ((int32_t)(l >> 20) + (int32_t)(r >> 20)) + 6 * (int32_t)(m >> 20);
-        pb = 5 * ((int32_t)(l & 0xFFFFFu) + (int32_t)(r & 0xFFFFFu)) + 6 * (int32_t)(m & 0xFFFFFu);
+        const uint32_t lr2 = l + r;  // field by field (lr_pack_ab)
+        pa = 5 * (int32_t)(lr2 >> kLrAbShift) + 6 * (int32_t)(m >> kLrAbShift);
+        pb = 5 * (int32_t)(lr2 & kLrAbMask) + 6 * (int32_t)(m & kLrAbMask);
     };
     int32_t sa0 = 0, ma0 = 0, sb0 = 0, mb0 = 0, sa1 = 0, ma1 = 0, sb1 = 0, mb1 = 0;
     if constexpr (r1 != 0) {

@@ -3,7 +3,8 @@
 The self-guided A/B step runs p * s and (256 - a2) * oneOverN * b on the
 24-bit multiplier and in 32 bits (lr.hpp, lr_ab), where the reference
 (loop_restoration.rs:640-661) computes in 64 bits with a saturating
-multiply; and the packed CDEF filter (cdef.hpp, cdef_filter_plane_pk) skips
+multiply, and packs A and B into one word whose neighbours are summed
+before the fields are split (lr_pack_ab); and the packed CDEF filter (cdef.hpp, cdef_filter_plane_pk) skips
 the final clip to the taps' min/max when a unit has only primary or only
 secondary taps (cdef.rs:592-640 always clips).  Both are exact only under
 the bounds below."""
@@ -72,6 +73,11 @@ def test_self_guided_products_fit_32_bits(bd):
             b = n * ((1 << bd) - 1)
             assert b < 1 << 17
             assert 255 * one_over_n * b < 1 << 32
+            # the packed A/B word (lr_pack_ab): A <= 256 at bit 22, B below;
+            # sums of three words must not carry from B into A
+            big_b = _round2(255 * one_over_n * b, 12)
+            assert big_b < 1 << 20
+            assert 3 * big_b < 1 << 22 and 3 * 256 < 1 << 10 and 256 << 22 < 1 << 32
 
 
 def _constrain(diff, threshold, damping):
