@@ -81,6 +81,32 @@ def test_cdef_parity(fi, si, gpu_calls, oracle_mod):
 
 
 @pytest.mark.parametrize("denom", [9, 12, 16])
+def _extreme_plane(shape, rng, maxv):
+    """Flat maxv regions (the largest self-guided B: a2 = 1 over flat
+    windows of the largest samples), 0/maxv checkerboards (the largest
+    variance, A = 256) and full-range noise, in bands of 24 rows."""
+    y = np.arange(shape[0])[:, None] // 24 % 3
+    x = np.arange(shape[1])[None, :]
+    checker = np.where((x + np.arange(shape[0])[:, None]) % 2 == 0, 0, maxv)
+    noise = rng.integers(0, maxv + 1, size=shape)
+    return np.where(y == 0, maxv, np.where(y == 1, checker, noise)).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", [0x5A01, 0x5A02])
+def test_loop_restoration_self_guided_extremes_12bit(seed, gpu_calls, oracle_mod):
+    """Self-guided units only, 12-bit, on CDEF and deblocked planes built
+    from the extremes above: the packed A/B words of neighbouring positions
+    are summed before their fields are split, which is exact only while B
+    stays below 2^20 (flat windows of 4095) and A at most 256."""
+    f = _frame(seed, lr_types=4, bit_depth=12, upscaled_width=260, frame_height=140)
+    deb, cdf = _lr_inputs(f, oracle_mod)
+    rng = np.random.default_rng(seed)
+    deb = [_extreme_plane(d.shape, rng, 4095) for d in deb]
+    cdf = [_extreme_plane(c.shape, rng, 4095) for c in cdf]
+    _eq(gpu_calls.loop_restoration_frame(f, deb, cdf),
+        oracle_mod.calls().loop_restoration_frame(f, deb, cdf), "loop_restoration_frame")
+
+
 @pytest.mark.parametrize("fi", [0, 1, 2, 3, 4, 6])
 def test_superres_parity(denom, fi, gpu_calls, oracle_mod):
     seed = 0x3F00 + denom * 7 + fi
