@@ -80,7 +80,18 @@ def test_cdef_parity(fi, si, gpu_calls, oracle_mod):
     _eq(got, want, "cdef_frame")
 
 
-@pytest.mark.parametrize("denom", [9, 12, 16])
+@pytest.mark.parametrize("seed", [0x5C01, 0x5C02])
+def test_cdef_extremes_12bit(seed, gpu_calls, oracle_mod):
+    """12-bit CDEF over flat, 0/4095 checkerboard and full-range noise bands
+    (the largest tap differences, |d| = 4095, and saturated constrain), every
+    unit filtered: the packed 16-bit filter holds differences, clamped taps
+    and their weighted sums in 16-bit halves."""
+    f = _frame(seed, bit_depth=12, cdef_skip_units=0, upscaled_width=200, frame_height=136)
+    rng = np.random.default_rng(seed)
+    src = [_extreme_plane(p.shape, rng, 4095) for p in f.planes]
+    _eq(gpu_calls.cdef_frame(f, src), oracle_mod.calls().cdef_frame(f, src), "cdef_frame")
+
+
 def _extreme_plane(shape, rng, maxv):
     """Flat maxv regions (the largest self-guided B: a2 = 1 over flat
     windows of the largest samples), 0/maxv checkerboards (the largest
@@ -107,6 +118,7 @@ def test_loop_restoration_self_guided_extremes_12bit(seed, gpu_calls, oracle_mod
         oracle_mod.calls().loop_restoration_frame(f, deb, cdf), "loop_restoration_frame")
 
 
+@pytest.mark.parametrize("denom", [9, 12, 16])
 @pytest.mark.parametrize("fi", [0, 1, 2, 3, 4, 6])
 def test_superres_parity(denom, fi, gpu_calls, oracle_mod):
     seed = 0x3F00 + denom * 7 + fi
