@@ -200,6 +200,31 @@ def test_film_grain_parity(fi, opts, gpu_calls, oracle_mod):
     _eq(got, want, "film_grain_synthesis")
 
 
+@pytest.mark.parametrize("clip", [0, 1], ids=["full_range", "clip_restricted"])
+def test_film_grain_extremes_12bit(clip, gpu_calls, oracle_mod):
+    """12-bit grain at full scaling over flat 0 / 4095, checkerboard and
+    full-range bands: every blend saturates against one end of the range
+    (full, or the restricted luma/chroma limits with clip_to_restricted_range)."""
+    f = _frame(0x5E01 + clip, enable_fg=1, fg_overlap=1, fg_ar_lag=3, fg_num_y_points=6,
+               fg_chroma_points=4, bit_depth=12, upscaled_width=200, frame_height=100)
+    d = f.desc
+    fg = d.fg
+    fg.clip_to_restricted_range = clip
+    for i in range(fg.num_y_points):
+        fg.point_y_scaling[i] = 255
+    for i in range(fg.num_cb_points):
+        fg.point_cb_scaling[i] = 255
+    for i in range(fg.num_cr_points):
+        fg.point_cr_scaling[i] = 255
+    rng = np.random.default_rng(0x5E01 + clip)
+    planes = [_extreme_plane(f.out_shape(p), rng, 4095) for p in range(f.num_planes)]
+    planes[0][:8] = 0  # a flat band at the low end as well
+    args = (fg, d.bit_depth, d.num_planes, d.subsampling_x, d.subsampling_y, d.matrix_coefficients)
+    want = oracle_mod.calls().film_grain_synthesis(*args, planes)
+    assert any(not np.array_equal(w, p) for w, p in zip(want, planes))
+    _eq(gpu_calls.film_grain_synthesis(*args, planes), want, "film_grain_synthesis")
+
+
 def test_film_grain_chroma_from_luma(gpu_calls, oracle_mod):
     f = _frame(0x5A11, enable_fg=1, fg_overlap=1, fg_ar_lag=3, fg_num_y_points=6,
                upscaled_width=130, frame_height=70)
